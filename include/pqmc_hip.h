@@ -1,0 +1,143 @@
+/*
+ * pqmc_hip.h -- C ABI of the MI355X projector-DQMC library (libpqmc_hip.so).
+ *
+ * The reference (LEOMMM816/PQMC_allinone) has no FFI: its boundary is the
+ * set of Fortran module subroutines the sweep loop of src/Main_PQMC.f90:81-156
+ * calls.  Each entry point below replaces one of them; a Fortran host binds
+ * them with iso_c_binding (pqmc_allinone_amd/fortran/pqmc_hip_mod.f90).
+ *
+ * Conventions: plain pointers and sizes only.  Complex numbers are pairs of
+ * doubles (re, im), matrices column-major as complex(dp) arrays in Fortran.
+ * `time` and `pf_id` are 1-based exactly as in the reference.  Device state
+ * (G, fields, B-string checkpoints, expKV table) is owned by the handle;
+ * host arrays are copied in or out only by the set_/get_ calls.
+ * Every function returns 0 on success or a pqmc_status code; the message of
+ * the last failure is available from pqmc_last_error().
+ */
+#ifndef PQMC_HIP_H
+#define PQMC_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pqmc_status {
+  PQMC_OK = 0,
+  PQMC_EINVAL = 1,     /* bad argument or call order */
+  PQMC_EHIP = 2,       /* HIP runtime error */
+  PQMC_ENUMERIC = 3,   /* non-finite Green's function or log-weight */
+  PQMC_ESYNC = 4,      /* a bounded device-side wait expired */
+  PQMC_ENODEV = 5      /* no usable gfx950 device */
+};
+
+typedef struct pqmc_handle pqmc_handle;
+
+/* Sizes of one chain.  L1 x L2 cells with two sites each (Ns = 2 L1 L2),
+ * four checkerboard bond families of L1 L2 / 2 bonds, Nb = 2 L1 L2 fields.
+ * Mirrors src/input.f90:167-205 (ntime, ngroup, nblock, nelec). */
+typedef struct pqmc_config {
+  int L1, L2;
+  int Lt;             /* ntime */
+  int ngroup;         /* stabilization interval */
+  int ne;             /* nelec */
+  int update_groups;  /* workgroups per family pass, 0 = automatic */
+} pqmc_config;
+
+/* Model tables as built by set_typed_pf / init_pf_para
+ * (src/mod_phonon_field.f90:249-421, src/input.f90:207-216). */
+typedef struct pqmc_model {
+  const int32_t *sites;  /* [4][npla][4] pla_site_list, 0-based sites */
+  const int32_t *bfi;    /* [4][npla]    bf_list, 0-based field index */
+  const int32_t *cross;  /* [4][npla]    boundary_crossing */
+  const double *bcph;    /* [4][npla][4x4] complex BC_phases, column-major */
+  const double *kcoe;    /* [4] complex K_coe (already times -hop) */
+  const double *vcoe;    /* [4] complex V_coe (already times ep_parameter) */
+  const double *k12;     /* [4][2x2] complex Kmatrix(1:2,3:4), column-major */
+  const double *v12;     /* [4][2x2] complex Vmatrix(1:2,3:4), column-major */
+  const double *bc;      /* [2] complex Lat%BC_phase */
+  double dtau, M, D, biased, jump_distance, hop, lam;
+  int32_t ncopy, tr_weight;
+} pqmc_model;
+
+/* Allocate a chain on the current device.  No reference counterpart (the
+ * reference allocates module globals in init / init_g_T0). */
+int pqmc_create(const pqmc_config *cfg, pqmc_handle **out);
+int pqmc_destroy(pqmc_handle *h);
+const char *pqmc_last_error(void);
+/* Derived sizes: Ns, Ncell, Nb, npla, nblock, total_width. */
+int pqmc_get_dims(const pqmc_handle *h, int32_t *out6);
+
+/* Model tables; replaces set_pf_main + init_pf_para (mod_phonon_field.f90:22-57). */
+int pqmc_set_model(pqmc_handle *h, const pqmc_model *m);
+/* boson_field(Nb, Lt) in/out (src/input.f90:86). */
+int pqmc_set_fields(pqmc_handle *h, const double *x);
+int pqmc_get_fields(pqmc_handle *h, double *x);
+/* Trial Slater matrix (Ns x ne complex); QDR on the device as
+ * set_nelec_occupied (src/Main_PQMC.f90:301-302). */
+int pqmc_set_trial(pqmc_handle *h, const double *slater);
+
+/* init_MC: init_expKV + set_pf_expKV_mat + init_g(start_time)
+ * (src/Main_PQMC.f90:185-203, src/mod_evolution.f90:46-111). */
+int pqmc_init_g(pqmc_handle *h, int start_time);
+
+/* update_phonon_field(ppf_list(pf_id), time) (src/mod_update.f90:13-201).
+ * u: 2*npla ran2 draws in call order (ran_sysm source, rands) per bond.
+ * acc (npla bytes, may be NULL) receives the accept flags.  The rank-4k
+ * change of G is applied lazily before the next operation that reads G. */
+int pqmc_update_family(pqmc_handle *h, int pf_id, int time, const float *u, uint8_t *acc);
+
+/* left_evolve / right_evolve on g_h with two_way = .true.
+ * (src/mod_evolution.f90:236-328): forward != 0 -> G <- E G E^-1, else
+ * G <- E^-1 G E, for family pf_id at slice time. */
+int pqmc_evolve_g(pqmc_handle *h, int time, int pf_id, int forward);
+
+/* left_evolve(time, mat(Ns,n), n, pf_id, two_way) / right_evolve(time,
+ * mat(n,Ns), ...) applied to a caller matrix (host memory, in place). */
+int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left, int two_way, double *mat, int n);
+
+/* get_g_scratch(time, forward, loop) (src/mod_evolution.f90:113-195).
+ * err_fast (may be NULL) receives max|G_fast - G_scratch|. */
+int pqmc_g_scratch(pqmc_handle *h, int time, int forward, double *err_fast);
+
+/* Meas_sys%take_measurement(time) + k_space_obs (src/mod_meas.f90:210-566)
+ * for a begin_measure call with meas_number = 0; out: total_width complex. */
+int pqmc_measure(pqmc_handle *h, int time, double *out);
+
+/* One iteration of the sweep loop body (src/Main_PQMC.f90:85-154) with the
+ * global update off: Lt slices x 4 families of updates and two-way wraps,
+ * get_g_scratch every ngroup slices, and (measure != 0) one measurement at
+ * time = Lt/2.  u: 2*Nb*Lt draws in call order.  ntc limits the number of
+ * time_count iterations (0 = all).  meas_out (total_width complex) may be
+ * NULL.  Accept flags stay on the device (pqmc_get_accepts). */
+int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, int measure, double *meas_out);
+
+/* State readers/writers (parity and checkpointing). */
+int pqmc_get_g(pqmc_handle *h, double *g);
+int pqmc_set_g(pqmc_handle *h, const double *g);
+int pqmc_get_qstring(pqmc_handle *h, int block, double *q);   /* Q_string(:,:,block) */
+int pqmc_get_dstring(pqmc_handle *h, double *d);              /* D_string(ne, nblock) */
+int pqmc_get_lncw(pqmc_handle *h, double *re_im);              /* ln_cw */
+int pqmc_get_accepts(pqmc_handle *h, uint8_t *acc, int64_t n); /* last sweep, call order */
+/* positive_accept, negative_accept, sync timeouts; err_fast_max. */
+int pqmc_get_counters(pqmc_handle *h, int64_t *out3, double *err_fast_max);
+int pqmc_synchronize(pqmc_handle *h);
+
+/* The ran2 generator of src/mod_ranstate.f90:171-191 seeded as
+ * init_rng_by_seed (:267-283); host-side, for drivers without their own. */
+typedef struct pqmc_ran2 { int32_t i, j, k, m, n; } pqmc_ran2;
+void pqmc_ran2_seed(pqmc_ran2 *s, int32_t seed);
+void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out);
+
+/* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */
+int pqmc_mfma_selftest(double *max_abs_err);
+
+/* Kernel timing hook for benchmarks: HIP events bracket every launch of
+ * the named kernel class on the handle's stream when enabled; returns the
+ * accumulated milliseconds and launch count. kind: 0 = family update,
+ * 1 = flush+wrap, 2 = stabilization (whole get_g_scratch). */
+int pqmc_timing(pqmc_handle *h, int enable, int kind, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,914 @@
+// C ABI and host orchestration of the projector-DQMC sweep on one MI355X.
+// One handle = one Markov chain; every call enqueues on the handle's stream
+// and returns without synchronizing unless it hands data back to the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pqmc_hip.h"
+#include "pqmc_common.h"
+
+// ---- kernel launchers (k_*.hip) ----
+struct upd_rec {
+  int p;
+  int pad[3];
+  zd X[16];
+};
+struct upd_args {
+  model_t md;
+  int f, t, P, chunk;
+  const int *fsites, *fbfi;
+  const zd *fph;
+  const int *fcross;
+  double *x;
+  ekv_t *ekv;
+  const float *u;
+  const zd *G;
+  zd *U, *VT;
+  uint8_t *acc;
+  upd_rec *recs;
+  unsigned long long *sync;
+  unsigned epoch;
+  int *rank_out;
+  double *lncw_acc;
+  unsigned long long *stats;
+};
+struct fw_args {
+  model_t md;
+  zd *G;
+  const zd *U, *VT;
+  const int *rank;
+  const int *pos_ffl;
+  const int *tsites;
+  int wrap;
+  const ekv_t *ekv_w;
+};
+struct prop_args {
+  model_t md;
+  zd *X;
+  int ncols;
+  const int *sites;
+  const ekv_t *ekv;
+  int t_first, nslices, backward, fam_only;
+};
+extern "C" {
+hipError_t pqmc_launch_update(const upd_args *a, hipStream_t s);
+hipError_t pqmc_launch_flush_wrap(const fw_args *a, hipStream_t s);
+hipError_t pqmc_launch_propagate(const prop_args *a, hipStream_t s);
+hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, const int *bfi, const int *cross,
+                                 const zd *ph, ekv_t *ekv, int t_lo, int t_hi, hipStream_t s);
+hipError_t pqmc_zgemm(int opA, int opB, int M, int N, int K, zd alpha, const zd *A, int lda, const zd *B, int ldb,
+                      zd beta, zd *C, int ldc, hipStream_t s);
+hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V, int ldv, zd *T, zd *tau, double *vn2,
+                         zd *beta_out, hipStream_t s);
+hipError_t pqmc_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int ldv, const zd *T, int nb,
+                           int trans, hipStream_t s);
+hipError_t pqmc_eye(zd *Q, int ldq, int m, int n, hipStream_t s);
+hipError_t pqmc_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int conjt, hipStream_t s);
+hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, hipStream_t s);
+hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *D, int nD,
+                     int tr_weight, int ncopy, zd *out, hipStream_t s);
+hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
+hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
+hipError_t pqmc_mfma_probe(const double *A, const double *B, double *C, hipStream_t s);
+hipError_t pqmc_launch_measure(const model_t *md, const zd *G, const double *x, int time, const zd *phase,
+                               zd *out, hipStream_t s);
+}
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PQMC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+struct qr_ws {  // workspace of one blocked QR
+  zd *V = nullptr, *T = nullptr, *tau = nullptr, *beta = nullptr;
+  double *vn2 = nullptr;
+};
+
+struct ev_pool {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+};
+
+struct pqmc_handle {
+  int L1, L2, Ncell, Ns, ne, Nb, npla, Lt, ngroup, nblock, total_width, P, nbq;
+  model_t md{};
+  hipStream_t st = nullptr;
+  bool model_set = false, trial_set = false, fields_set = false, g_ready = false;
+  // device model tables
+  int *d_sites = nullptr, *d_bfi = nullptr, *d_cross = nullptr, *d_pos = nullptr;
+  zd *d_ph = nullptr;
+  double *d_x = nullptr;
+  ekv_t *d_ekv = nullptr;
+  // Green's function and delayed-update factors
+  zd *d_G = nullptr, *d_Gn = nullptr, *d_U = nullptr, *d_VT = nullptr;
+  // B-string
+  zd *d_Q = nullptr, *d_D = nullptr, *d_R = nullptr;
+  zd *d_slQ = nullptr, *d_slD = nullptr, *d_slR = nullptr;
+  zd *d_W = nullptr, *d_L = nullptr, *d_M = nullptr, *d_QM = nullptr, *d_Z = nullptr;
+  qr_ws qa, qm;
+  // update chain state
+  float *d_u = nullptr;
+  size_t u_cap = 0;
+  uint8_t *d_acc = nullptr;
+  upd_rec *d_recs = nullptr;
+  unsigned long long *d_sync = nullptr;
+  unsigned epoch = 0;
+  int *d_rank = nullptr;
+  double *d_lncw_acc = nullptr;
+  zd *d_lncw = nullptr;
+  unsigned long long *d_stats = nullptr, *d_err = nullptr;
+  zd *d_meas = nullptr, *d_kphase = nullptr;
+  int pending = -1;  // family whose flush is pending
+  double err_fast_max = 1e-6;
+  // timing
+  bool timing[3] = {false, false, false};
+  ev_pool pools[3];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans[3];
+};
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+static hipEvent_t pool_get(ev_pool &p) {
+  if (p.used == p.ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p.ev.push_back(e);
+  }
+  return p.ev[p.used++];
+}
+
+struct span_guard {  // HIP events around one kernel launch when timing is on
+  pqmc_handle *h;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  span_guard(pqmc_handle *hh, int k) : h(hh), kind(k) {
+    if (h->timing[kind]) {
+      a = pool_get(h->pools[kind]);
+      b = pool_get(h->pools[kind]);
+      if (a && b) hipEventRecord(a, h->st);
+    }
+  }
+  ~span_guard() {
+    if (a && b) {
+      hipEventRecord(b, h->st);
+      h->spans[kind].push_back({a, b});
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+
+extern "C" const char *pqmc_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
+  if (!cfg || !out) return fail(PQMC_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->L1 < 2 || cfg->L2 < 2 || (cfg->L1 * cfg->L2) % 2 || cfg->Lt < 1 || cfg->ngroup < 1 || cfg->ne < 1)
+    return fail(PQMC_EINVAL, "invalid sizes");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PQMC_ENODEV, "no HIP device");
+  pqmc_handle *h = new pqmc_handle();
+  h->L1 = cfg->L1;
+  h->L2 = cfg->L2;
+  h->Ncell = cfg->L1 * cfg->L2;
+  h->Ns = 2 * h->Ncell;
+  h->ne = cfg->ne;
+  h->Nb = 2 * h->Ncell;
+  h->npla = h->Ns / 4;
+  h->Lt = cfg->Lt;
+  h->ngroup = cfg->ngroup;
+  h->nblock = cfg->Lt / cfg->ngroup + 2;
+  h->total_width = 3 + 19 * 2 * h->Ncell;
+  if (h->ne > h->Ns) { delete h; return fail(PQMC_EINVAL, "ne > Ns"); }
+  if (h->Ns > 1024) { delete h; return fail(PQMC_EINVAL, "Ns > 1024 not supported"); }
+  int P = cfg->update_groups;
+  if (P <= 0) P = std::min(8, std::max(1, (h->npla + 24) / 25));
+  while ((h->npla + P - 1) / P > 64) ++P;
+  h->P = P;
+  h->nbq = (h->Ns <= 512) ? 16 : 8;
+  const int Ns = h->Ns, ne = h->ne, npla = h->npla;
+  h->md.Ns = Ns; h->md.ne = ne; h->md.npla = npla; h->md.Nb = h->Nb; h->md.Lt = h->Lt;
+  h->md.Ncell = h->Ncell; h->md.L1 = h->L1; h->md.L2 = h->L2;
+#define A_(p, n)                                                                   \
+  if (dalloc(&(p), (n)) != hipSuccess) {                                           \
+    pqmc_destroy(h);                                                               \
+    return fail(PQMC_EHIP, "device allocation failed: " #p);                       \
+  }
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(PQMC_EHIP, "stream creation failed");
+  }
+  A_(h->d_sites, 4 * npla * 4);
+  A_(h->d_bfi, 4 * npla);
+  A_(h->d_cross, 4 * npla);
+  A_(h->d_pos, 4 * Ns);
+  A_(h->d_ph, 4 * npla * 4);
+  A_(h->d_x, (size_t)h->Nb * h->Lt);
+  A_(h->d_ekv, (size_t)h->Lt * 4 * npla);
+  A_(h->d_G, (size_t)Ns * Ns);
+  A_(h->d_Gn, (size_t)Ns * Ns);
+  A_(h->d_U, (size_t)Ns * Ns);
+  A_(h->d_VT, (size_t)Ns * Ns);
+  A_(h->d_Q, (size_t)Ns * ne * h->nblock);
+  A_(h->d_D, (size_t)ne * h->nblock);
+  A_(h->d_R, (size_t)ne * ne * h->nblock);
+  A_(h->d_slQ, (size_t)Ns * ne);
+  A_(h->d_slD, ne);
+  A_(h->d_slR, (size_t)ne * ne);
+  A_(h->d_W, (size_t)Ns * ne);
+  A_(h->d_L, (size_t)Ns * ne);
+  A_(h->d_M, (size_t)ne * ne);
+  A_(h->d_QM, (size_t)ne * ne);
+  A_(h->d_Z, (size_t)ne * Ns);
+  A_(h->qa.V, (size_t)Ns * ne);
+  A_(h->qa.T, (size_t)16 * 16 * ((ne + 7) / 8));
+  A_(h->qa.tau, ne);
+  A_(h->qa.beta, ne);
+  A_(h->qa.vn2, ne);
+  A_(h->qm.V, (size_t)ne * ne);
+  A_(h->qm.T, (size_t)16 * 16 * ((ne + 7) / 8));
+  A_(h->qm.tau, ne);
+  A_(h->qm.beta, ne);
+  A_(h->qm.vn2, ne);
+  A_(h->d_acc, (size_t)h->Nb * h->Lt);
+  A_(h->d_recs, npla);
+  A_(h->d_sync, 2);
+  A_(h->d_rank, 1);
+  A_(h->d_lncw_acc, 1);
+  A_(h->d_lncw, 1);
+  A_(h->d_stats, 4);
+  A_(h->d_err, 1);
+  A_(h->d_meas, h->total_width);
+  A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
+#undef A_
+  if (hipMemsetAsync(h->d_sync, 0, 2 * sizeof(unsigned long long), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_acc, 0, (size_t)h->Nb * h->Lt, h->st) != hipSuccess) {
+    pqmc_destroy(h);
+    return fail(PQMC_EHIP, "initial memset failed");
+  }
+  // k-space phases exp(i FBZ(k).r_i) (src/mod_lattice.f90:164-205, unit square lattice)
+  {
+    std::vector<zd> ph((size_t)h->Ncell * h->Ncell);
+    const double two_pi = 2.0 * std::acos(-1.0);
+    double kx0 = (h->L1 % 2) * two_pi / h->L1 / 2.0 - 0.5 * two_pi;
+    double ky0 = (h->L2 % 2) * two_pi / h->L2 / 2.0 - 0.5 * two_pi;
+    for (int k = 0; k < h->Ncell; ++k) {
+      double f0 = two_pi / h->L1 * (k % h->L1) + kx0, f1 = two_pi / h->L2 * (k / h->L1) + ky0;
+      for (int i = 0; i < h->Ncell; ++i) {
+        double arg = f0 * (i % h->L1) + f1 * (i / h->L1);
+        ph[k + (size_t)h->Ncell * i] = make_double2(std::cos(arg), std::sin(arg));
+      }
+    }
+    if (hipMemcpyAsync(h->d_kphase, ph.data(), sizeof(zd) * ph.size(), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
+      pqmc_destroy(h);
+      return fail(PQMC_EHIP, "phase upload failed");
+    }
+  }
+  *out = h;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_destroy(pqmc_handle *h) {
+  if (!h) return PQMC_OK;
+  if (h->st) hipStreamSynchronize(h->st);
+  void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_U,
+                  h->d_VT, h->d_Q, h->d_D, h->d_R, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
+                  h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
+                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_recs, h->d_sync, h->d_rank, h->d_lncw_acc,
+                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (auto &pool : h->pools)
+    for (auto e : pool.ev) hipEventDestroy(e);
+  if (h->st) hipStreamDestroy(h->st);
+  delete h;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_dims(const pqmc_handle *h, int32_t *o) {
+  if (!h || !o) return fail(PQMC_EINVAL, "null argument");
+  o[0] = h->Ns; o[1] = h->Ncell; o[2] = h->Nb; o[3] = h->npla; o[4] = h->nblock; o[5] = h->total_width;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_model(pqmc_handle *h, const pqmc_model *m) {
+  if (!h || !m || !m->sites || !m->bfi || !m->cross || !m->bcph || !m->kcoe || !m->vcoe || !m->k12 || !m->v12 ||
+      !m->bc)
+    return fail(PQMC_EINVAL, "null argument");
+  const int npla = h->npla, Ns = h->Ns;
+  std::vector<int> pos(4 * Ns, -1);
+  for (int f = 0; f < 4; ++f)
+    for (int q = 0; q < npla * 4; ++q) {
+      int s = m->sites[f * npla * 4 + q];
+      if (s < 0 || s >= Ns || pos[f * Ns + s] != -1) return fail(PQMC_EINVAL, "bond family does not partition the sites");
+      pos[f * Ns + s] = q;
+    }
+  for (int q = 0; q < 4 * npla; ++q)
+    if (m->bfi[q] < 0 || m->bfi[q] >= h->Nb) return fail(PQMC_EINVAL, "field index out of range");
+  std::vector<zd> ph(4 * npla * 4);
+  for (int fb = 0; fb < 4 * npla; ++fb)
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) {
+        const double *p = &m->bcph[2 * (fb * 16 + a + 4 * (2 + c))];
+        ph[fb * 4 + a + 2 * c] = make_double2(p[0], p[1]);
+      }
+  model_t &md = h->md;
+  md.dtau = m->dtau; md.M = m->M; md.D = m->D; md.biased = m->biased; md.jump = m->jump_distance;
+  md.hop = m->hop; md.lam = m->lam; md.ncopy = m->ncopy; md.tr_weight = m->tr_weight;
+  for (int f = 0; f < 4; ++f) {
+    md.kcoe[f] = make_double2(m->kcoe[2 * f], m->kcoe[2 * f + 1]);
+    md.vcoe[f] = make_double2(m->vcoe[2 * f], m->vcoe[2 * f + 1]);
+    for (int q = 0; q < 4; ++q) {
+      md.k12[f][q] = make_double2(m->k12[2 * (4 * f + q)], m->k12[2 * (4 * f + q) + 1]);
+      md.v12[f][q] = make_double2(m->v12[2 * (4 * f + q)], m->v12[2 * (4 * f + q) + 1]);
+    }
+  }
+  md.bc[0] = make_double2(m->bc[0], m->bc[1]);
+  md.bc[1] = make_double2(m->bc[2], m->bc[3]);
+  if (md.ncopy < 1) return fail(PQMC_EINVAL, "ncopy < 1");
+  HIPCHK(hipMemcpyAsync(h->d_sites, m->sites, sizeof(int) * 4 * npla * 4, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_bfi, m->bfi, sizeof(int) * 4 * npla, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_cross, m->cross, sizeof(int) * 4 * npla, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_pos, pos.data(), sizeof(int) * 4 * Ns, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_ph, ph.data(), sizeof(zd) * ph.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->model_set = true;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_fields(pqmc_handle *h, const double *x) {
+  if (!h || !x) return fail(PQMC_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(h->d_x, x, sizeof(double) * h->Nb * h->Lt, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->fields_set = true;
+  h->g_ready = false;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_fields(pqmc_handle *h, double *x) {
+  if (!h || !x) return fail(PQMC_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(x, h->d_x, sizeof(double) * h->Nb * h->Lt, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// blocked Householder QR of A (m x n, lda): Q (m x n, ld m) explicit,
+// D = diag(R), Rout = D^-1 R (or its conjugate transpose).
+
+static int run_qr(pqmc_handle *h, qr_ws &w, zd *A, int lda, int m, int n, zd *Q, zd *D, zd *Rout, int conjt) {
+  const int nb = h->nbq;
+  const int np = (n + nb - 1) / nb;
+  for (int p = 0; p < np; ++p) {
+    const int j0 = p * nb, nbp = std::min(nb, n - j0);
+    HIPCHK(pqmc_qr_panel(A, lda, m, j0, nbp, w.V, m, w.T + (size_t)256 * p, w.tau + j0, w.vn2 + j0, w.beta + j0,
+                         h->st));
+    if (j0 + nbp < n)
+      HIPCHK(pqmc_apply_refl(A + j0 + (size_t)lda * (j0 + nbp), lda, m - j0, n - j0 - nbp, w.V + j0 + (size_t)m * j0,
+                             m, w.T + (size_t)256 * p, nbp, 1, h->st));
+  }
+  if (Q) {
+    HIPCHK(pqmc_eye(Q, m, m, n, h->st));
+    for (int p = np - 1; p >= 0; --p) {
+      const int j0 = p * nb, nbp = std::min(nb, n - j0);
+      HIPCHK(pqmc_apply_refl(Q + j0 + (size_t)m * j0, m, m - j0, n - j0, w.V + j0 + (size_t)m * j0, m,
+                             w.T + (size_t)256 * p, nbp, 0, h->st));
+    }
+  }
+  if (D && Rout) HIPCHK(pqmc_qdr_finish(A, lda, n, Rout, D, conjt, h->st));
+  return PQMC_OK;
+}
+
+// G = QR (QL^H QR)^-1 QL^H with ln_cw (cal_g_and_det_T0, src/mod_evolution.f90:330-357).
+// QL is the left B-string stored as lmat^H (Ns x ne); QR is the right one.
+static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout) {
+  const int Ns = h->Ns, ne = h->ne;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // M = QL^H QR
+  HIPCHK(pqmc_zgemm(1, 0, ne, ne, Ns, one, QL, Ns, QRm, Ns, zero, h->d_M, ne, h->st));
+  int rc = run_qr(h, h->qm, h->d_M, ne, ne, ne, h->d_QM, nullptr, nullptr, 0);
+  if (rc) return rc;
+  // Z = QM^H QL^H  (ne x Ns)
+  HIPCHK(pqmc_zgemm(1, 1, ne, Ns, ne, one, h->d_QM, ne, QL, Ns, zero, h->d_Z, ne, h->st));
+  // Z <- R_M^-1 Z
+  HIPCHK(pqmc_trsm_upper(h->d_M, ne, ne, h->d_Z, ne, Ns, h->st));
+  // G = QR Z
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRm, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
+  HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_D, ne * h->nblock, h->md.tr_weight, h->md.ncopy,
+                   h->d_lncw, h->st));
+  return PQMC_OK;
+}
+
+static zd *QS(pqmc_handle *h, int block) { return h->d_Q + (size_t)h->Ns * h->ne * (block - 1); }
+static zd *DS(pqmc_handle *h, int block) { return h->d_D + (size_t)h->ne * (block - 1); }
+static zd *RS(pqmc_handle *h, int block) { return h->d_R + (size_t)h->ne * h->ne * (block - 1); }
+
+static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backward) {
+  prop_args a;
+  a.md = h->md;
+  a.X = X;
+  a.ncols = h->ne;
+  a.sites = h->d_sites;
+  a.ekv = h->d_ekv;
+  a.t_first = t_first;
+  a.nslices = nslices;
+  a.backward = backward;
+  a.fam_only = -1;
+  HIPCHK(pqmc_launch_propagate(&a, h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_trial(pqmc_handle *h, const double *slater) {
+  if (!h || !slater) return fail(PQMC_EINVAL, "null argument");
+  const size_t n = (size_t)h->Ns * h->ne;
+  HIPCHK(hipMemcpyAsync(h->d_W, slater, sizeof(zd) * n, hipMemcpyHostToDevice, h->st));
+  int rc = run_qr(h, h->qa, h->d_W, h->Ns, h->Ns, h->ne, h->d_slQ, h->d_slD, h->d_slR, 0);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->trial_set = true;
+  return PQMC_OK;
+}
+
+// pending delayed update flushed into G, fused with an optional two-way wrap
+static int flush_wrap(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
+  if (h->pending < 0 && wrap == 0) return PQMC_OK;
+  fw_args a;
+  a.md = h->md;
+  a.G = h->d_G;
+  a.U = h->d_U;
+  a.VT = h->d_VT;
+  a.rank = (h->pending >= 0) ? h->d_rank : nullptr;
+  a.pos_ffl = (h->pending >= 0) ? h->d_pos + (size_t)h->pending * h->Ns : nullptr;
+  const int ftile = wrap ? f_wrap : h->pending;
+  a.tsites = h->d_sites + (size_t)ftile * h->npla * 4;
+  a.wrap = wrap;
+  a.ekv_w = wrap ? h->d_ekv + ((size_t)(t_wrap - 1) * 4 + f_wrap) * h->npla : nullptr;
+  {
+    span_guard sg(h, 1);
+    HIPCHK(pqmc_launch_flush_wrap(&a, h->st));
+  }
+  h->pending = -1;
+  return PQMC_OK;
+}
+
+static int flush(pqmc_handle *h) { return flush_wrap(h, 0, 0, 0); }
+
+extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (!h->model_set || !h->trial_set || !h->fields_set) return fail(PQMC_EINVAL, "model, trial and fields must be set");
+  if (start_time < 1 || start_time > h->Lt) return fail(PQMC_EINVAL, "start_time out of range");
+  const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, nb = h->nblock;
+  h->pending = -1;
+  HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_x, h->d_bfi, h->d_cross, h->d_ph, h->d_ekv, 1, h->Lt, h->st));
+  const size_t qb = sizeof(zd) * Ns * ne;
+  HIPCHK(hipMemcpyAsync(QS(h, 1), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(QS(h, nb), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(DS(h, 1), h->d_slD, sizeof(zd) * ne, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(DS(h, nb), h->d_slD, sizeof(zd) * ne, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(RS(h, 1), h->d_slR, sizeof(zd) * ne * ne, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(RS(h, nb), h->d_slR, sizeof(zd) * ne * ne, hipMemcpyDeviceToDevice, h->st));
+  const int start_block = start_time / ng + 1;
+  const int stb = (start_block - 1) * ng + 1;
+  HIPCHK(hipMemcpyAsync(h->d_W, QS(h, start_block), qb, hipMemcpyDeviceToDevice, h->st));
+  const zd *rmat = h->d_W;
+  int p = stb;
+  while (p <= h->Lt) {
+    int pend = std::min(h->Lt, ((p - 1) / ng + 1) * ng);
+    int rc = propagate(h, h->d_W, p, pend - p + 1, 0);
+    if (rc) return rc;
+    rmat = h->d_W;
+    if (pend % ng == 0) {
+      const int block = (pend - 1) / ng + 2;
+      rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
+      if (rc) return rc;
+      rmat = QS(h, block);
+      if (pend < h->Lt) HIPCHK(hipMemcpyAsync(h->d_W, rmat, qb, hipMemcpyDeviceToDevice, h->st));
+    }
+    p = pend + 1;
+  }
+  int rc = cal_g(h, QS(h, nb), rmat, h->d_G);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->g_ready = true;
+  return PQMC_OK;
+}
+
+static int ensure_u(pqmc_handle *h, size_t n) {
+  if (n <= h->u_cap) return PQMC_OK;
+  if (h->d_u) hipFree(h->d_u);
+  h->d_u = nullptr;
+  HIPCHK(dalloc(&h->d_u, n));
+  h->u_cap = n;
+  return PQMC_OK;
+}
+
+// launch one family pass; d_u points at its 2*npla draws on the device
+static int launch_update(pqmc_handle *h, int f, int t, const float *d_u, uint8_t *d_acc) {
+  int rc = flush(h);
+  if (rc) return rc;
+  upd_args a;
+  a.md = h->md;
+  a.f = f;
+  a.t = t;
+  a.P = h->P;
+  a.chunk = (h->npla + h->P - 1) / h->P;
+  a.fsites = h->d_sites + (size_t)f * h->npla * 4;
+  a.fbfi = h->d_bfi + (size_t)f * h->npla;
+  a.fph = h->d_ph + (size_t)f * h->npla * 4;
+  a.fcross = h->d_cross + (size_t)f * h->npla;
+  a.x = h->d_x;
+  a.ekv = h->d_ekv;
+  a.u = d_u;
+  a.G = h->d_G;
+  a.U = h->d_U;
+  a.VT = h->d_VT;
+  a.acc = d_acc;
+  a.recs = h->d_recs;
+  a.sync = h->d_sync;
+  a.epoch = ++h->epoch;
+  a.rank_out = h->d_rank;
+  a.lncw_acc = h->d_lncw_acc;
+  a.stats = h->d_stats;
+  {
+    span_guard sg(h, 0);
+    HIPCHK(pqmc_launch_update(&a, h->st));
+  }
+  h->pending = f;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_update_family(pqmc_handle *h, int pf_id, int time, const float *u, uint8_t *acc) {
+  if (!h || !u) return fail(PQMC_EINVAL, "null argument");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  if (pf_id < 1 || pf_id > 4 || time < 1 || time > h->Lt) return fail(PQMC_EINVAL, "pf_id or time out of range");
+  int rc = ensure_u(h, 2 * (size_t)h->npla);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h->d_u, u, sizeof(float) * 2 * h->npla, hipMemcpyHostToDevice, h->st));
+  rc = launch_update(h, pf_id - 1, time, h->d_u, h->d_acc);
+  if (rc) return rc;
+  if (acc) {
+    HIPCHK(hipMemcpyAsync(acc, h->d_acc, h->npla, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_evolve_g(pqmc_handle *h, int time, int pf_id, int forward) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  if (pf_id < 1 || pf_id > 4 || time < 1 || time > h->Lt) return fail(PQMC_EINVAL, "pf_id or time out of range");
+  return flush_wrap(h, forward ? 1 : 2, pf_id - 1, time);
+}
+
+extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left, int two_way, double *mat, int n) {
+  if (!h || !mat) return fail(PQMC_EINVAL, "null argument");
+  if (pf_id < 1 || pf_id > 4 || time < 1 || time > h->Lt || n < 1) return fail(PQMC_EINVAL, "argument out of range");
+  if (two_way && n != h->Ns) return fail(PQMC_EINVAL, "two-way evolve needs n == Ns");
+  if (n > h->Ns) return fail(PQMC_EINVAL, "n > Ns");
+  const int Ns = h->Ns;
+  // ensure the expKV table exists (init_g builds it)
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  int rc = flush(h);
+  if (rc) return rc;
+  if (two_way) {
+    // run the two-way wrap on a scratch copy in d_Gn
+    HIPCHK(hipMemcpyAsync(h->d_Gn, mat, sizeof(zd) * Ns * Ns, hipMemcpyHostToDevice, h->st));
+    fw_args a;
+    a.md = h->md;
+    a.G = h->d_Gn;
+    a.U = a.VT = nullptr;
+    a.rank = nullptr;
+    a.pos_ffl = nullptr;
+    a.tsites = h->d_sites + (size_t)(pf_id - 1) * h->npla * 4;
+    a.wrap = left ? 1 : 2;
+    a.ekv_w = h->d_ekv + ((size_t)(time - 1) * 4 + (pf_id - 1)) * h->npla;
+    HIPCHK(pqmc_launch_flush_wrap(&a, h->st));
+    HIPCHK(hipMemcpyAsync(mat, h->d_Gn, sizeof(zd) * Ns * Ns, hipMemcpyDeviceToHost, h->st));
+  } else {
+    // one family: left: mat(Ns,n) <- E mat; right: mat(n,Ns) <- mat E = (E mat^H)^H
+    // (E is Hermitian)
+    zd *X = h->d_Gn;
+    if (left) {
+      HIPCHK(hipMemcpyAsync(X, mat, sizeof(zd) * Ns * n, hipMemcpyHostToDevice, h->st));
+    } else {
+      HIPCHK(hipMemcpyAsync(h->d_U, mat, sizeof(zd) * Ns * n, hipMemcpyHostToDevice, h->st));
+      HIPCHK(pqmc_conjt(h->d_U, n, n, Ns, X, Ns, h->st));
+    }
+    prop_args a;
+    a.md = h->md;
+    a.X = X;
+    a.ncols = n;
+    a.sites = h->d_sites;
+    a.ekv = h->d_ekv;
+    a.t_first = time;
+    a.nslices = 1;
+    a.backward = 0;
+    a.fam_only = pf_id - 1;
+    HIPCHK(pqmc_launch_propagate(&a, h->st));
+    if (left) {
+      HIPCHK(hipMemcpyAsync(mat, X, sizeof(zd) * Ns * n, hipMemcpyDeviceToHost, h->st));
+    } else {
+      HIPCHK(pqmc_conjt(X, Ns, Ns, n, h->d_U, n, h->st));
+      HIPCHK(hipMemcpyAsync(mat, h->d_U, sizeof(zd) * Ns * n, hipMemcpyDeviceToHost, h->st));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+// get_g_scratch_T0 (src/mod_evolution.f90:113-195)
+static int g_scratch(pqmc_handle *h, int time, int forward) {
+  const int Ns = h->Ns, ne = h->ne, ng = h->ngroup;
+  int rc = flush(h);
+  if (rc) return rc;
+  span_guard sg(h, 2);
+  const size_t qb = sizeof(zd) * Ns * ne;
+  const zd *lmatH, *rmat;
+  if (forward) {
+    const int block = time / ng + 1;
+    if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
+    HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block - 1), qb, hipMemcpyDeviceToDevice, h->st));
+    rc = propagate(h, h->d_W, time - ng, ng, 0);
+    if (rc) return rc;
+    rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
+    if (rc) return rc;
+    lmatH = QS(h, block + 1);
+    rmat = QS(h, block);
+  } else {
+    int block = time / ng + 2;
+    if (ng == 1) block -= 1;
+    if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
+    HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block + 1), qb, hipMemcpyDeviceToDevice, h->st));
+    rc = propagate(h, h->d_W, time + ng - 1, ng, 1);
+    if (rc) return rc;
+    rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 1);
+    if (rc) return rc;
+    lmatH = QS(h, block);
+    rmat = QS(h, block - 1);
+  }
+  rc = cal_g(h, lmatH, rmat, h->d_Gn);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(unsigned long long), h->st));
+  HIPCHK(pqmc_maxdiff(h->d_G, h->d_Gn, (long)Ns * Ns, h->d_err, h->st));
+  std::swap(h->d_G, h->d_Gn);
+  return PQMC_OK;
+}
+
+static int read_err(pqmc_handle *h, double *err) {
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, h->d_err, sizeof(bits), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double e;
+  std::memcpy(&e, &bits, sizeof(e));
+  if (err) *err = e;
+  if (e > h->err_fast_max) h->err_fast_max = e;
+  if (!std::isfinite(e)) return fail(PQMC_ENUMERIC, "non-finite Green's function after stabilization");
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_g_scratch(pqmc_handle *h, int time, int forward, double *err_fast) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  int rc = g_scratch(h, time, forward);
+  if (rc) return rc;
+  return read_err(h, err_fast);
+}
+
+extern "C" int pqmc_measure(pqmc_handle *h, int time, double *out) {
+  if (!h || !out) return fail(PQMC_EINVAL, "null argument");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  if (time < 1 || time >= h->Lt) return fail(PQMC_EINVAL, "measurement time must be in [1, Lt-1]");
+  int rc = flush(h);
+  if (rc) return rc;
+  HIPCHK(pqmc_launch_measure(&h->md, h->d_G, h->d_x, time, h->d_kphase, h->d_meas, h->st));
+  HIPCHK(hipMemcpyAsync(out, h->d_meas, sizeof(zd) * h->total_width, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, int measure, double *meas_out) {
+  if (!h || !u) return fail(PQMC_EINVAL, "null argument");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  const int Lt = h->Lt, npla = h->npla;
+  if (ntc <= 0 || ntc > Lt) ntc = Lt;
+  const size_t nu = 2 * (size_t)h->Nb * Lt;
+  int rc = ensure_u(h, nu);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h->d_u, u, sizeof(float) * nu, hipMemcpyHostToDevice, h->st));
+  size_t off = 0;
+  bool measured = false;
+  for (int tc = 1; tc <= ntc; ++tc) {
+    int time;
+    if (!forward) {
+      time = Lt - tc + 1;
+      for (int f = 3; f >= 0; --f) {
+        rc = flush_wrap(h, 2, f, time);
+        if (rc) return rc;
+        rc = launch_update(h, f, time, h->d_u + off, h->d_acc + off / 2);
+        if (rc) return rc;
+        off += 2 * (size_t)npla;
+      }
+    } else {
+      time = tc;
+      for (int f = 0; f < 4; ++f) {
+        rc = launch_update(h, f, time, h->d_u + off, h->d_acc + off / 2);
+        if (rc) return rc;
+        rc = flush_wrap(h, 1, f, time);
+        if (rc) return rc;
+        off += 2 * (size_t)npla;
+      }
+    }
+    if (tc % h->ngroup == 0) {
+      rc = g_scratch(h, forward ? time + 1 : time, forward);
+      if (rc) return rc;
+    }
+    if (measure && time == Lt / 2) {
+      rc = flush(h);
+      if (rc) return rc;
+      HIPCHK(pqmc_launch_measure(&h->md, h->d_G, h->d_x, time, h->d_kphase, h->d_meas, h->st));
+      measured = true;
+    }
+  }
+  rc = flush(h);
+  if (rc) return rc;
+  if (measured && meas_out)
+    HIPCHK(hipMemcpyAsync(meas_out, h->d_meas, sizeof(zd) * h->total_width, hipMemcpyDeviceToHost, h->st));
+  if (ntc >= h->ngroup) {
+    rc = read_err(h, nullptr);
+    if (rc) return rc;
+  }
+  unsigned long long stats[4];
+  HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in the update chain");
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_g(pqmc_handle *h, double *g) {
+  if (!h || !g) return fail(PQMC_EINVAL, "null argument");
+  int rc = flush(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(g, h->d_G, sizeof(zd) * h->Ns * h->Ns, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_g(pqmc_handle *h, const double *g) {
+  if (!h || !g) return fail(PQMC_EINVAL, "null argument");
+  h->pending = -1;
+  HIPCHK(hipMemcpyAsync(h->d_G, g, sizeof(zd) * h->Ns * h->Ns, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_qstring(pqmc_handle *h, int block, double *q) {
+  if (!h || !q) return fail(PQMC_EINVAL, "null argument");
+  if (block < 1 || block > h->nblock) return fail(PQMC_EINVAL, "block out of range");
+  HIPCHK(hipMemcpyAsync(q, QS(h, block), sizeof(zd) * h->Ns * h->ne, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_dstring(pqmc_handle *h, double *d) {
+  if (!h || !d) return fail(PQMC_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(d, h->d_D, sizeof(zd) * h->ne * h->nblock, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_lncw(pqmc_handle *h, double *ri) {
+  if (!h || !ri) return fail(PQMC_EINVAL, "null argument");
+  zd v;
+  HIPCHK(hipMemcpyAsync(&v, h->d_lncw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  ri[0] = v.x;
+  ri[1] = v.y;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_accepts(pqmc_handle *h, uint8_t *acc, int64_t n) {
+  if (!h || !acc) return fail(PQMC_EINVAL, "null argument");
+  if (n < 0 || n > (int64_t)h->Nb * h->Lt) return fail(PQMC_EINVAL, "n out of range");
+  HIPCHK(hipMemcpyAsync(acc, h->d_acc, (size_t)n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
+  if (!h || !o3) return fail(PQMC_EINVAL, "null argument");
+  unsigned long long s[4];
+  HIPCHK(hipMemcpyAsync(s, h->d_stats, sizeof(s), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  o3[0] = (int64_t)s[0];
+  o3[1] = (int64_t)s[1];
+  o3[2] = (int64_t)s[2];
+  if (efm) *efm = h->err_fast_max;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_synchronize(pqmc_handle *h) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_timing(pqmc_handle *h, int enable, int kind, double *ms, int64_t *launches) {
+  if (!h || kind < 0 || kind > 2) return fail(PQMC_EINVAL, "bad argument");
+  HIPCHK(hipStreamSynchronize(h->st));
+  double tot = 0.0;
+  for (auto &pr : h->spans[kind]) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, pr.first, pr.second));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (launches) *launches = (int64_t)h->spans[kind].size();
+  h->spans[kind].clear();
+  h->pools[kind].used = 0;
+  h->timing[kind] = enable != 0;
+  return PQMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ran2 (src/mod_ranstate.f90:171-191) and init_rng_by_seed (:267-283)
+
+extern "C" void pqmc_ran2_seed(pqmc_ran2 *s, int32_t seed) {
+  int32_t v[5] = {seed, seed, seed, seed, seed};
+  for (int i = 0; i < 3; ++i)
+    if (v[i] < 0) v[i] = ~v[i];
+  for (int i = 3; i < 5; ++i)
+    if (v[i] == 0) v[i] = 1;
+  s->i = v[0]; s->j = v[1]; s->k = v[2]; s->m = v[3]; s->n = v[4];
+}
+
+extern "C" void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out) {
+  // amm = nearest(1.0_sp,-1.0_sp)/(-2^31)
+  const float amm = -0x1.fffffep-32f;
+  uint32_t I = (uint32_t)s->i, J = (uint32_t)s->j, K = (uint32_t)s->k, M = (uint32_t)s->m, N = (uint32_t)s->n;
+  for (int64_t q = 0; q < n; ++q) {
+    int32_t r = (int32_t)(I - K);
+    if (r < 0) r = (int32_t)((uint32_t)r + 2147483579u);
+    I = J;
+    J = K;
+    K = (uint32_t)r;
+    N ^= N << 13;
+    N ^= N >> 17;
+    N ^= N << 5;
+    const uint32_t lo = M & 65535u;
+    M = ((3533u * (M >> 16) + lo) << 16) + 3533u * lo + 820265819u;
+    const int32_t v = (int32_t)((N ^ K) + M);
+    out[q] = amm * (float)((v < 0) ? v : ~v);
+  }
+  s->i = (int32_t)I; s->j = (int32_t)J; s->k = (int32_t)K; s->m = (int32_t)M; s->n = (int32_t)N;
+}
+
+extern "C" int pqmc_mfma_selftest(double *max_abs_err) {
+  double A[64], B[64], C[256], ref[256];
+  for (int i = 0; i < 64; ++i) {
+    A[i] = std::sin(0.37 * i + 0.1);
+    B[i] = std::cos(0.23 * i - 0.4);
+  }
+  for (int j = 0; j < 16; ++j)
+    for (int i = 0; i < 16; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < 4; ++k) s += A[i + 16 * k] * B[k + 4 * j];
+      ref[i + 16 * j] = s;
+    }
+  double *dA, *dB, *dC;
+  HIPCHK(hipMalloc(&dA, sizeof(A)));
+  HIPCHK(hipMalloc(&dB, sizeof(B)));
+  HIPCHK(hipMalloc(&dC, sizeof(C)));
+  HIPCHK(hipMemcpy(dA, A, sizeof(A), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, sizeof(B), hipMemcpyHostToDevice));
+  HIPCHK(pqmc_mfma_probe(dA, dB, dC, nullptr));
+  HIPCHK(hipMemcpy(C, dC, sizeof(C), hipMemcpyDeviceToHost));
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dC);
+  double m = 0.0;
+  for (int i = 0; i < 256; ++i) m = std::fmax(m, std::fabs(C[i] - ref[i]));
+  if (max_abs_err) *max_abs_err = m;
+  return PQMC_OK;
+}

@@ -1,0 +1,240 @@
+"""Host-side mirror of the reference's sweep interfaces over the C ABI.
+
+The names and argument meanings follow the Fortran modules the sweep loop of
+src/Main_PQMC.f90:81-156 calls:
+
+===========================  ============================================
+reference                    here
+===========================  ============================================
+init_MC / init_g(1)          ``Chain.init_g(start_time)``
+update_phonon_field(ppf,t)   ``Chain.update_phonon_field(pf_id, time, u)``
+left_evolve(t,g_h,..,.true.) ``Chain.left_evolve(time, pf_id)``
+right_evolve(t,g_h,..)       ``Chain.right_evolve(time, pf_id)``
+get_g_scratch(t,fwd,loop)    ``Chain.get_g_scratch(time, forward)``
+Meas_sys%begin_measure       ``Measurements.begin_measure(chain, fwd, t)``
+loop body (one sweep)        ``Chain.sweep(forward, draws, measure)``
+ran2_s / rands / ran_sysm    ``Ran2`` (src/mod_ranstate.f90:171-236)
+===========================  ============================================
+
+All state lives on the GPU; ``g_h``, fields and B-string checkpoints are
+copied to the host only when read.  Errors raise ``PQMCError`` where the
+reference would ``stop``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .model import Model
+
+# observable registry of input/model/EPSOCZ.nml:262-283 (field kinds get a _k twin)
+SCALARS = ("BF_KE", "BF_PE", "BF_X")
+FIELDS = ("BFx_BFx", "BFx_BFy", "BFy_BFy", "BFxy_BFxy", "Jxx_Jxx", "Jxy_Jxy", "Jyx_Jyx", "Jyy_Jyy",
+          "Jxx_Jyx", "Jxy_Jyy", "Jxx_Jyy", "Jxy_Jyx", "cJxy_cJxy", "cJyx_cJyx", "den_den", "SC_SC",
+          "Sz_Sz", "Sx_Sx", "Sy_Sy")
+
+
+def observable_layout(ncell: int):
+    """(name, lo, hi) offsets as ms_finalize_layout (src/mod_meas.f90:691-707), 0-based."""
+    out, cur = [], 0
+    for n in SCALARS:
+        out.append((n, cur, cur + 1))
+        cur += 1
+    for n in FIELDS:
+        out.append((n, cur, cur + ncell))
+        cur += ncell
+        out.append((n + "_k", cur, cur + ncell))
+        cur += ncell
+    return out
+
+
+class Ran2:
+    """ran2 of src/mod_ranstate.f90:171-191, seeded as init_rng_by_seed (:267-283)."""
+
+    def __init__(self, seed: int):
+        self.state = _lib.Ran2()
+        _lib.lib().pqmc_ran2_seed(C.byref(self.state), int(seed))
+
+    def draws(self, n: int) -> np.ndarray:
+        out = np.empty(int(n), np.float32)
+        _lib.lib().pqmc_ran2_fill(C.byref(self.state), int(n), _lib.ptr(out))
+        return out
+
+    def rands(self) -> float:
+        return float(self.draws(1)[0])
+
+    def ran_sysm(self) -> float:
+        r = self.draws(1)[0]
+        return float(np.float32(r * np.float32(2) - np.float32(1)))
+
+
+class Chain:
+    """One projector-DQMC Markov chain resident on the GPU."""
+
+    def __init__(self, model: Model, update_groups: int = 0):
+        self.model = model
+        p = model.p
+        self._L = _lib.lib()
+        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(update_groups))
+        h = C.c_void_p()
+        _lib.check(self._L.pqmc_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = [np.ascontiguousarray(model.sites, np.int32), np.ascontiguousarray(model.bfi, np.int32),
+                      np.ascontiguousarray(model.cross, np.int32),
+                      np.ascontiguousarray(np.transpose(model.bcph, (0, 1, 3, 2)), np.complex128),
+                      np.ascontiguousarray(model.kcoe), np.ascontiguousarray(model.vcoe),
+                      np.ascontiguousarray(np.transpose(model.k12, (0, 2, 1))),
+                      np.ascontiguousarray(np.transpose(model.v12, (0, 2, 1))),
+                      np.ascontiguousarray(model.bc)]
+        k = self._keep
+        mt = _lib.ModelTables(*[_lib.ptr(a) for a in k], p.dtau, model.M, model.D, p.biased,
+                              model.jump_distance, p.hop, p.lam, int(p.ncopy), int(bool(p.tr_weight)))
+        _lib.check(self._L.pqmc_set_model(self.h, C.byref(mt)))
+        dims = np.zeros(6, np.int32)
+        _lib.check(self._L.pqmc_get_dims(self.h, _lib.ptr(dims)))
+        self.Ns, self.Ncell, self.Nb, self.npla, self.nblock, self.total_width = (int(v) for v in dims)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pqmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- inputs -------------------------------------------------------
+    def set_fields(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, np.float64)
+        if x.size != self.Nb * self.model.p.Lt:
+            raise ValueError("fields must hold Nb*Lt values")
+        _lib.check(self._L.pqmc_set_fields(self.h, _lib.ptr(x)))
+
+    def get_fields(self) -> np.ndarray:
+        x = np.empty((self.model.p.Lt, self.Nb), np.float64)
+        _lib.check(self._L.pqmc_get_fields(self.h, _lib.ptr(x)))
+        return x
+
+    def set_trial(self, slater: np.ndarray):
+        s = np.asfortranarray(slater, np.complex128)
+        if s.shape != (self.Ns, self.model.p.ne):
+            raise ValueError("slater must be Ns x ne")
+        _lib.check(self._L.pqmc_set_trial(self.h, _lib.ptr(s)))
+
+    # ---- reference interfaces ----------------------------------------
+    def init_g(self, start_time: int = 1):
+        _lib.check(self._L.pqmc_init_g(self.h, int(start_time)))
+
+    def update_phonon_field(self, pf_id: int, time: int, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, np.float32)
+        if u.size != 2 * self.npla:
+            raise ValueError("need 2*npla draws")
+        acc = np.zeros(self.npla, np.uint8)
+        _lib.check(self._L.pqmc_update_family(self.h, int(pf_id), int(time), _lib.ptr(u), _lib.ptr(acc)))
+        return acc
+
+    def left_evolve(self, time: int, pf_id: int):
+        _lib.check(self._L.pqmc_evolve_g(self.h, int(time), int(pf_id), 1))
+
+    def right_evolve(self, time: int, pf_id: int):
+        _lib.check(self._L.pqmc_evolve_g(self.h, int(time), int(pf_id), 0))
+
+    def evolve_matrix(self, time: int, pf_id: int, mat: np.ndarray, left: bool, two_way: bool) -> np.ndarray:
+        """left_evolve(time, mat(Ns,n)) / right_evolve(time, mat(n,Ns)) on a host matrix."""
+        m = np.asfortranarray(mat, np.complex128).copy(order="F")
+        n = m.shape[1] if left else m.shape[0]
+        _lib.check(self._L.pqmc_evolve_matrix(self.h, int(time), int(pf_id), int(bool(left)), int(bool(two_way)),
+                                              _lib.ptr(m), int(n)))
+        return m
+
+    def get_g_scratch(self, time: int, forward: bool) -> float:
+        err = C.c_double(0.0)
+        _lib.check(self._L.pqmc_g_scratch(self.h, int(time), int(bool(forward)), C.addressof(err)))
+        return err.value
+
+    def take_measurement(self, time: int) -> np.ndarray:
+        out = np.empty(self.total_width, np.complex128)
+        _lib.check(self._L.pqmc_measure(self.h, int(time), _lib.ptr(out)))
+        return out
+
+    def sweep(self, forward: bool, u: np.ndarray, measure: bool = False, ntc: int = 0):
+        """One iteration of the reference's `do loop` body; returns the measurement or None."""
+        u = np.ascontiguousarray(u, np.float32)
+        if u.size != 2 * self.Nb * self.model.p.Lt:
+            raise ValueError("need 2*Nb*Lt draws")
+        out = np.empty(self.total_width, np.complex128) if measure else None
+        _lib.check(self._L.pqmc_sweep(self.h, int(bool(forward)), _lib.ptr(u), int(ntc), int(bool(measure)),
+                                      _lib.ptr(out) if out is not None else None))
+        return out
+
+    # ---- state -------------------------------------------------------
+    @property
+    def g_h(self) -> np.ndarray:
+        g = np.empty((self.Ns, self.Ns), np.complex128, order="F")
+        _lib.check(self._L.pqmc_get_g(self.h, _lib.ptr(g)))
+        return g
+
+    @g_h.setter
+    def g_h(self, g: np.ndarray):
+        g = np.asfortranarray(g, np.complex128)
+        _lib.check(self._L.pqmc_set_g(self.h, _lib.ptr(g)))
+
+    @property
+    def ln_cw(self) -> complex:
+        v = np.zeros(2)
+        _lib.check(self._L.pqmc_get_lncw(self.h, _lib.ptr(v)))
+        return complex(v[0], v[1])
+
+    def q_string(self, block: int) -> np.ndarray:
+        q = np.empty((self.Ns, self.model.p.ne), np.complex128, order="F")
+        _lib.check(self._L.pqmc_get_qstring(self.h, int(block), _lib.ptr(q)))
+        return q
+
+    def d_string(self) -> np.ndarray:
+        d = np.empty((self.model.p.ne, self.nblock), np.complex128, order="F")
+        _lib.check(self._L.pqmc_get_dstring(self.h, _lib.ptr(d)))
+        return d
+
+    def accepts(self, n: int | None = None) -> np.ndarray:
+        n = self.Nb * self.model.p.Lt if n is None else int(n)
+        a = np.empty(n, np.uint8)
+        _lib.check(self._L.pqmc_get_accepts(self.h, _lib.ptr(a), n))
+        return a
+
+    def counters(self):
+        o = np.zeros(3, np.int64)
+        efm = C.c_double(0.0)
+        _lib.check(self._L.pqmc_get_counters(self.h, _lib.ptr(o), C.addressof(efm)))
+        return {"positive_accept": int(o[0]), "negative_accept": int(o[1]), "sync_timeouts": int(o[2]),
+                "err_fast_max": efm.value}
+
+    def synchronize(self):
+        _lib.check(self._L.pqmc_synchronize(self.h))
+
+    def timing(self, kind: int, enable: bool):
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        _lib.check(self._L.pqmc_timing(self.h, int(bool(enable)), int(kind), C.addressof(ms), C.addressof(n)))
+        return ms.value, n.value
+
+
+class Measurements:
+    """Binning of begin_measure / compute_one_bin_mean (src/mod_meas.f90:729-763, 847-858)."""
+
+    def __init__(self, chain: Chain, nmeas_per_bin: int):
+        self.chain = chain
+        self.nmeas = int(nmeas_per_bin)
+        self.cur = []
+        self.bin_means = []
+
+    def begin_measure(self, forward: bool, time: int):
+        data = self.chain.take_measurement(time)
+        self.cur.append(data)
+        if len(self.cur) == self.nmeas:
+            self.bin_means.append(np.mean(np.real(np.array(self.cur)), axis=0))
+            self.cur = []
+        return data

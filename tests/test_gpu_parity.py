@@ -1,0 +1,134 @@
+"""GPU path vs the oracle on the same seeded inputs (C ABI through ctypes).
+
+Tolerances: the plain path is FP64; G, Q_string and observables must agree
+to 1e-10 relative (max|a-b| / max|b|); accept/reject sequences must be
+identical.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make_pair(L=4, Lt=100, ngroup=10, ne=16, seed=1234, lam=0.64, P=0):
+    from pqmc_allinone_amd.host import Chain
+    p = ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ngroup, ne=ne, lam=lam)
+    m = build_model(p)
+    x, sl = synthetic_inputs(m, seed)
+    orc = O.Oracle(L, L, Lt, ngroup, ne, lam=lam)
+    orc.set_fields(x)
+    orc.set_trial(sl)
+    orc.init(1)
+    ch = Chain(m, update_groups=P)
+    ch.set_fields(x)
+    ch.set_trial(sl)
+    ch.init_g(1)
+    return m, orc, ch
+
+
+def test_mfma_layout():
+    from pqmc_allinone_amd import _lib
+    import ctypes as C
+    e = C.c_double(-1.0)
+    _lib.check(_lib.lib().pqmc_mfma_selftest(C.addressof(e)))
+    assert e.value < 1e-13
+
+
+def test_init_g_matches_oracle():
+    m, orc, ch = make_pair()
+    assert rel(ch.g_h, orc.g) < TOL
+    for blk in (1, 2, 5, m.nblock - 1, m.nblock):
+        assert rel(ch.q_string(blk), orc.q_string(blk)) < TOL, blk
+    assert rel(ch.d_string(), orc.d_string()) < TOL
+    lw_g, lw_o = ch.ln_cw, orc.ln_cw
+    assert abs(lw_g.real - lw_o.real) < 1e-9 * max(1.0, abs(lw_o.real))
+    assert abs(np.exp(1j * lw_g.imag) - np.exp(1j * lw_o.imag)) < 1e-8
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_two_way_wrap_matches_oracle(forward):
+    m, orc, ch = make_pair()
+    for f in (1, 2, 3, 4):
+        if forward:
+            ch.left_evolve(7, f)
+        else:
+            ch.right_evolve(7, f)
+        orc.wrap_g(7, f, forward)
+    assert rel(ch.g_h, orc.g) < TOL
+
+
+def test_one_way_evolve_matrix_matches_oracle():
+    m, orc, ch = make_pair()
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((m.Ns, m.p.ne)) + 1j * rng.standard_normal((m.Ns, m.p.ne))
+    for f in (1, 3):
+        got = ch.evolve_matrix(11, f, a, left=True, two_way=False)
+        ref = orc.left_evolve(11, a, f, 0)
+        assert rel(got, ref) < TOL
+        b = a.T.copy()
+        got = ch.evolve_matrix(11, f, b, left=False, two_way=False)
+        ref = orc.right_evolve(11, b, f, 0)
+        assert rel(got, ref) < TOL
+
+
+@pytest.mark.parametrize("P", [1, 2])
+def test_update_family_matches_oracle(P):
+    m, orc, ch = make_pair(P=P)
+    rng = O.Ran2Stream(99)
+    for t, f in [(1, 1), (1, 2), (2, 3), (50, 4)]:
+        u = rng.draws(2 * m.npla)
+        acc_g = ch.update_phonon_field(f, t, u)
+        acc_o = orc.update_phonon_field(f, t, u)
+        assert np.array_equal(acc_g, acc_o), (t, f)
+        assert rel(ch.g_h, orc.g) < TOL, (t, f)
+    assert np.array_equal(ch.get_fields(), orc.get_fields())
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_g_scratch_matches_oracle(forward):
+    m, orc, ch = make_pair()
+    time = 11 if forward else 91
+    e_g = ch.get_g_scratch(time, forward)
+    e_o = orc.get_g_scratch(time, forward)
+    assert rel(ch.g_h, orc.g) < TOL
+    # G sat at Lt+1 before the call, so err_fast is large; both must agree on it
+    assert abs(e_g - e_o) < 1e-9 * max(1.0, e_o)
+    assert rel(ch.q_string(time // 10 + (1 if forward else 2)), orc.q_string(time // 10 + (1 if forward else 2))) < TOL
+
+
+def test_measurement_matches_oracle():
+    m, orc, ch = make_pair()
+    a = ch.take_measurement(m.p.Lt // 2)
+    b = orc.measure(m.p.Lt // 2)
+    assert rel(a, b) < TOL
+
+
+@pytest.mark.parametrize("P", [0, 1])
+def test_sweeps_match_oracle(P):
+    """Identical accept/reject sequence and G over alternating sweeps (C1)."""
+    m, orc, ch = make_pair(P=P)
+    rng = O.Ran2Stream(1234)
+    forward = True
+    for sweep in range(4):
+        forward = not forward  # the first sweep runs backward (Main_PQMC.f90:85)
+        u = rng.draws(2 * m.Nb * m.p.Lt)
+        mg = ch.sweep(forward, u, measure=True)
+        acc_o, mo = orc.sweep(forward, u, measure=True)
+        acc_g = ch.accepts()
+        assert np.array_equal(acc_g, acc_o), f"sweep {sweep}: first mismatch at {np.argmax(acc_g != acc_o)}"
+        assert rel(ch.g_h, orc.g) < TOL, sweep
+        assert rel(mg, mo) < TOL, sweep
+    assert np.array_equal(ch.get_fields(), orc.get_fields())
+    c = ch.counters()
+    assert c["sync_timeouts"] == 0
+    pa, na, _ = orc.counters()
+    assert (c["positive_accept"], c["negative_accept"]) == (pa, na)
