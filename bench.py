@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Chain-sweeps/s of the projector-DQMC sweep on one MI355X.
+
+Workload (default): 14x14 EPSOCZ lattice (Ns = 392, ne = 196), Lt = 1000,
+dtau = 0.1, stabilization every 10 slices, coupling 0.64, omega = 1, fields
+N(0,1) from seed 7, one chain, plain FP64.  One step = one chain-sweep: one
+iteration of the reference's sweep loop (all slices x 4 bond families x all
+bonds, wraps, QR stabilization every 10 slices, one measurement at Lt/2).
+Directions alternate starting backward, as in the reference.
+
+Prints one JSON line.  `roofline` is that of the dominant kernel class,
+timed with HIP events on the chain's own stream over the timed region;
+`cpu_baseline` times the oracle (a single-threaded C restatement of the
+reference algorithm) on a bounded slice of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix)
+PEAK_HBM = 8.0e12    # MI355X HBM3E
+
+CONFIGS = {
+    # name: (L, Lt, ngroup, ne, seed)
+    "c2": (14, 1000, 10, 196, 7),
+    "c1": (4, 100, 10, 16, 1234),
+}
+
+
+def sweep_flops(Ns, ne, Nb, Lt, ng, n_acc, d=4):
+    """F_sweep of the survey's accounting: rank-4 updates + wraps + propagation + stabilization."""
+    return (8 * d * Ns ** 2 * n_acc + Lt * Nb * (16 * d * d * Ns + 8 * d * d * ne)
+            + (Lt / ng) * (32 * Ns * ne ** 2 + (16 / 3) * ne ** 3 + 8 * Ns ** 2 * ne))
+
+
+def cpu_baseline(L, Lt, ng, ne, seed, target_s):
+    """The oracle on the first slices of a backward sweep, scaled to chain-sweeps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne))
+    x, sl = synthetic_inputs(m, seed)
+    orc = O.Oracle(L, L, Lt, ng, ne)
+    orc.set_fields(x)
+    orc.set_trial(sl)
+    orc.init(1)
+    rng = O.Ran2Stream(seed)
+    u = rng.draws(2 * m.Nb * Lt)
+    # calibrate on one stabilization group, then run a bounded sample
+    t0 = time.perf_counter()
+    orc.sweep(False, u, ntc=ng)
+    t_group = time.perf_counter() - t0
+    ngroups = max(1, min(Lt // ng - 1, int(target_s / max(t_group, 1e-9))))
+    orc2 = O.Oracle(L, L, Lt, ng, ne)
+    orc2.set_fields(x)
+    orc2.set_trial(sl)
+    orc2.init(1)
+    t0 = time.perf_counter()
+    orc2.sweep(False, u, ntc=ngroups * ng)
+    dt = time.perf_counter() - t0
+    frac = ngroups * ng / Lt
+    return {"value": frac / dt, "unit": "chain-sweeps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C, 1 thread) over the first {ngroups * ng} of {Lt} slices of a backward sweep "
+                      f"({dt:.1f} s), scaled by slices"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--update-groups", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs with no "
+              "exchange during sweeps, so multi-GPU throughput is N x the one-GPU value", file=sys.stderr)
+        sys.exit(2)
+
+    from pqmc_allinone_amd.host import Chain, Ran2
+    from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+
+    L, Lt, ng, ne, seed = CONFIGS[args.config]
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne, lam=0.64, omega=1.0))
+    x, sl = synthetic_inputs(m, seed)
+    ch = Chain(m, update_groups=args.update_groups)
+    ch.set_fields(x)
+    ch.set_trial(sl)
+    ch.init_g(1)
+    rng = Ran2(seed)
+    nd = 2 * m.Nb * Lt
+    forward = True
+    for _ in range(args.warmup):
+        forward = not forward
+        ch.sweep(forward, rng.draws(nd), measure=True)
+    c0 = ch.counters()
+    timing = not args.no_timing
+    if timing:
+        for k in range(3):
+            ch.timing(k, True)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        forward = not forward
+        ch.sweep(forward, rng.draws(nd), measure=True)
+    ch.synchronize()
+    dt = time.perf_counter() - t0
+    kern = {}
+    if timing:
+        for k, name in enumerate(("update_chain", "flush_wrap", "stabilization")):
+            ms, n = ch.timing(k, False)
+            kern[name] = {"ms_total": ms, "launches": n, "us_avg": 1e3 * ms / max(n, 1)}
+    c1 = ch.counters()
+    n_acc = (c1["positive_accept"] - c0["positive_accept"]) / args.steps
+    n_prop = n_acc + (c1["negative_accept"] - c0["negative_accept"]) / args.steps
+    sps = args.steps / dt
+    Ns, Nb = m.Ns, m.Nb
+    F = sweep_flops(Ns, ne, Nb, Lt, ng, n_acc)
+    roof = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k]["ms_total"])
+        if dom == "flush_wrap":
+            # rank-4k flush (8 Ns^2 r flops, r = 4 accepts of the family) + two-way wrap (64 Ns^2 flops)
+            fl = (8 * Ns ** 2 * 4 * n_acc + Lt * 4 * 64 * Ns ** 2) / max(kern[dom]["launches"] / args.steps, 1)
+            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
+                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        elif dom == "update_chain":
+            # delayed-update column work of the chain per family pass: for accept j (rank 4j before it),
+            # 2 x Ns x 4 x 4j complex MACs (8 flops each); accepts per pass = n_acc / (4 Lt)
+            a = n_acc / (4 * Lt)
+            fl = 8 * 2 * Ns * 4 * 4 * (a * (a - 1) / 2)
+            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
+                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        else:
+            fl = (32 * Ns * ne ** 2 + (16 / 3) * ne ** 3 + 8 * Ns ** 2 * ne)
+            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
+                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        roof["traffic"] = None
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
+    out = {
+        "metric": "chain-sweeps/sec at 14x14 Lt=1000 (plain FP64 single chain)",
+        "value": sps,
+        "unit": "chain-sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic: fields N(0,1) seed %d, trial = lowest ne eigenvectors of H0 + 1e-3 random hopping" % seed,
+        "config": {"workload": f"EPSOCZ {L}x{L} Lt={Lt} dtau=0.1 ngroup={ng} lambda=0.64 omega=1 Ns={Ns} "
+                               f"ne={ne} 1 chain, plain FP64, measure at Lt/2", "chains": 1},
+        "roofline": roof,
+        "sweep_roofline": {"flops_per_sweep": F, "achieved_tflops": F * sps / 1e12,
+                           "frac_fp64": F * sps / PEAK_FP64},
+        "acceptance": n_acc / max(n_prop, 1),
+        "kernels": kern,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_core": (sps / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,12 @@ typedef struct pqmc_ran2 { int32_t i, j, k, m, n; } pqmc_ran2;
 void pqmc_ran2_seed(pqmc_ran2 *s, int32_t seed);
 void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out);
 
+/* Debug phase timers of the family update kernel: enable allocates and
+ * zeroes 8 accumulators per workgroup; out (8*P doubles, microseconds) and
+ * P may be NULL. Phases: set-up, wait, decisions, own-row updates, fill-in,
+ * later accepts.  No reference counterpart (diagnostics). */
+int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P);
+
 /* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */
 int pqmc_mfma_selftest(double *max_abs_err);
 

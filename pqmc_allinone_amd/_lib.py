@@ -69,6 +69,7 @@ SIGNATURES = {
     "pqmc_ran2_fill": (None, [C.POINTER(Ran2), C.c_int64, _P]),
     "pqmc_mfma_selftest": (_I, [_P]),
     "pqmc_timing": (_I, [_P, _I, _I, _P, _P]),
+    "pqmc_update_profile": (_I, [_P, _I, _P, _P]),
 }
 
 _lib = None

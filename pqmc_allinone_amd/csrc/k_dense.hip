@@ -219,10 +219,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
 extern "C" hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V, int ldv, zd *T,
                                     zd *tau, double *vn2, zd *beta_out, hipStream_t s) {
   const size_t lds = sizeof(zd) * (size_t)(m - j0) * nb;
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_qr_panel, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    attr = true;
+  static size_t attr = 48 * 1024;
+  if (lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_qr_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
   }
   hipLaunchKernelGGL(k_qr_panel, dim3(1), dim3(QR_THREADS), lds, s, A, lda, m, j0, nb, V, ldv, T, tau, vn2,
                      beta_out);
@@ -369,10 +370,11 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const zd *R, int ldr, int n,
 
 extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, hipStream_t s) {
   const size_t lds = sizeof(zd) * (size_t)n * (TS_COLS + TS_KB);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-    attr = true;
+  static size_t attr = 48 * 1024;
+  if (lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
   }
   hipLaunchKernelGGL(k_trsm_upper, dim3((ncols + TS_COLS - 1) / TS_COLS), dim3(256), lds, s, R, ldr, n, Z, ldz,
                      ncols);

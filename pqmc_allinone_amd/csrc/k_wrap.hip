@@ -17,7 +17,7 @@
 struct fw_args {
   model_t md;
   zd *G;
-  const zd *U, *VT;     // flush factors, family-order rows of the flush family
+  const zd *U, *VT;     // flush factors (row-major, ld Ns), family-order rows of the flush family
   const int *rank;      // device rank r (null: no flush)
   const int *pos_ffl;   // [Ns] site -> family-order row of the flush family
   const int *tsites;    // [npla*4] sites of the tiling family
@@ -54,11 +54,11 @@ __global__ __launch_bounds__(256) void k_flush_wrap(fw_args A) {
       const int q = q0 + k;
       zd v = zmk(0.0, 0.0);
       if (which == 0) {
-        if (q < r && ii < nr) v = A.U[rpos[ii] + (size_t)Ns * q];
+        if (q < r && ii < nr) v = A.U[(size_t)rpos[ii] * Ns + q];
         sAr[k][ii] = v.x;
         sAi[k][ii] = v.y;
       } else {
-        if (q < r && ii < nc) v = A.VT[cpos[ii] + (size_t)Ns * q];
+        if (q < r && ii < nc) v = A.VT[(size_t)cpos[ii] * Ns + q];
         sBr[k][ii] = v.x;
         sBi[k][ii] = v.y;
       }

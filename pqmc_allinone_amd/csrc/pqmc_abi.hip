@@ -17,6 +17,7 @@
 struct upd_rec {
   int p;
   int pad[3];
+  zd relec;
   zd X[16];
 };
 struct upd_args {
@@ -37,6 +38,7 @@ struct upd_args {
   int *rank_out;
   double *lncw_acc;
   unsigned long long *stats;
+  unsigned long long *prof;
 };
 struct fw_args {
   model_t md;
@@ -134,6 +136,7 @@ struct pqmc_handle {
   unsigned long long *d_stats = nullptr, *d_err = nullptr;
   zd *d_meas = nullptr, *d_kphase = nullptr;
   int pending = -1;  // family whose flush is pending
+  unsigned long long *d_prof = nullptr;  // update-chain phase timers (debug)
   double err_fast_max = 1e-6;
   // timing
   bool timing[3] = {false, false, false};
@@ -201,7 +204,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   if (h->Ns > 1024) { delete h; return fail(PQMC_EINVAL, "Ns > 1024 not supported"); }
   int P = cfg->update_groups;
   if (P <= 0) P = std::min(8, std::max(1, (h->npla + 24) / 25));
-  while ((h->npla + P - 1) / P > 64) ++P;
+  while ((h->npla + P - 1) / P > 128) ++P;
   h->P = P;
   h->nbq = (h->Ns <= 512) ? 16 : 8;
   const int Ns = h->Ns, ne = h->ne, npla = h->npla;
@@ -297,7 +300,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->d_VT, h->d_Q, h->d_D, h->d_R, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_recs, h->d_sync, h->d_rank, h->d_lncw_acc,
-                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase};
+                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -551,6 +554,7 @@ static int launch_update(pqmc_handle *h, int f, int t, const float *d_u, uint8_t
   a.rank_out = h->d_rank;
   a.lncw_acc = h->d_lncw_acc;
   a.stats = h->d_stats;
+  a.prof = h->d_prof;
   {
     span_guard sg(h, 0);
     HIPCHK(pqmc_launch_update(&a, h->st));
@@ -825,6 +829,30 @@ extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
   o3[1] = (int64_t)s[1];
   o3[2] = (int64_t)s[2];
   if (efm) *efm = h->err_fast_max;
+  return PQMC_OK;
+}
+
+// Phase timers of the update chain (debug): enable != 0 allocates and
+// zeroes them; out (8*P values, may be NULL) receives microseconds.
+extern "C" int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (P) *P = h->P;
+  if (out && h->d_prof) {
+    std::vector<unsigned long long> v(8 * h->P);
+    HIPCHK(hipMemcpyAsync(v.data(), h->d_prof, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    int rate = 0;
+    HIPCHK(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, 0));
+    for (size_t i = 0; i < v.size(); ++i) out[i] = rate ? 1e3 * (double)v[i] / rate : (double)v[i];
+  }
+  if (enable) {
+    if (!h->d_prof) HIPCHK(dalloc(&h->d_prof, 8 * h->P));
+    HIPCHK(hipMemsetAsync(h->d_prof, 0, sizeof(unsigned long long) * 8 * h->P, h->st));
+  } else if (h->d_prof) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipFree(h->d_prof));
+    h->d_prof = nullptr;
+  }
   return PQMC_OK;
 }
 
