@@ -188,6 +188,14 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
     return fail(PQMC_EINVAL, "invalid sizes");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PQMC_ENODEV, "no HIP device");
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+      return fail(PQMC_ENODEV, "cannot query the current HIP device");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return fail(PQMC_ENODEV, std::string("current device is ") + prop.gcnArchName + ", the library is built for gfx950");
+  }
   pqmc_handle *h = new pqmc_handle();
   h->L1 = cfg->L1;
   h->L2 = cfg->L2;
@@ -203,7 +211,9 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   if (h->ne > h->Ns) { delete h; return fail(PQMC_EINVAL, "ne > Ns"); }
   if (h->Ns > 1024) { delete h; return fail(PQMC_EINVAL, "Ns > 1024 not supported"); }
   int P = cfg->update_groups;
-  if (P <= 0) P = std::min(8, std::max(1, (h->npla + 24) / 25));
+  // about 6 bonds per workgroup: per-accept row work is latency bound, so
+  // more workgroups sharing the rows win until handoffs dominate
+  if (P <= 0) P = std::min(16, std::max(1, (h->npla + 5) / 6));
   while ((h->npla + P - 1) / P > 128) ++P;
   h->P = P;
   h->nbq = (h->Ns <= 512) ? 16 : 8;
