@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--update-groups", type=int, default=0)
+    ap.add_argument("--block-bonds", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
@@ -95,7 +95,7 @@ def main():
     L, Lt, ng, ne, seed = CONFIGS[args.config]
     m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne, lam=0.64, omega=1.0))
     x, sl = synthetic_inputs(m, seed)
-    ch = Chain(m, update_groups=args.update_groups)
+    ch = Chain(m, block_bonds=args.block_bonds)
     ch.set_fields(x)
     ch.set_trial(sl)
     ch.init_g(1)

@@ -40,7 +40,7 @@ typedef struct pqmc_config {
   int Lt;             /* ntime */
   int ngroup;         /* stabilization interval */
   int ne;             /* nelec */
-  int update_groups;  /* workgroups per family pass, 0 = automatic */
+  int block_bonds;    /* bonds per decision block of a family pass (1..8), 0 = automatic */
 } pqmc_config;
 
 /* Model tables as built by set_typed_pf / init_pf_para
@@ -82,8 +82,7 @@ int pqmc_init_g(pqmc_handle *h, int start_time);
 
 /* update_phonon_field(ppf_list(pf_id), time) (src/mod_update.f90:13-201).
  * u: 2*npla ran2 draws in call order (ran_sysm source, rands) per bond.
- * acc (npla bytes, may be NULL) receives the accept flags.  The rank-4k
- * change of G is applied lazily before the next operation that reads G. */
+ * acc (npla bytes, may be NULL) receives the accept flags. */
 int pqmc_update_family(pqmc_handle *h, int pf_id, int time, const float *u, uint8_t *acc);
 
 /* left_evolve / right_evolve on g_h with two_way = .true.
@@ -128,10 +127,12 @@ typedef struct pqmc_ran2 { int32_t i, j, k, m, n; } pqmc_ran2;
 void pqmc_ran2_seed(pqmc_ran2 *s, int32_t seed);
 void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out);
 
-/* Debug phase timers of the family update kernel: enable allocates and
- * zeroes 8 accumulators per workgroup; out (8*P doubles, microseconds) and
- * P may be NULL. Phases: set-up, wait, decisions, own-row updates, fill-in,
- * later accepts.  No reference counterpart (diagnostics). */
+/* Debug phase timers of the family-pass kernel: enable allocates and zeroes
+ * 8 accumulators per workgroup; out (8*P doubles, microseconds) and P (the
+ * workgroup count) may be NULL.  Workgroup 0 (decisions): set-up, decisions,
+ * wait for the next block's tiles, replay.  Others (column owners): load and
+ * wrap, wait, block-step set-up, update and publish, final wrap and store.
+ * No reference counterpart (diagnostics). */
 int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P);
 
 /* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */
@@ -140,7 +141,8 @@ int pqmc_mfma_selftest(double *max_abs_err);
 /* Kernel timing hook for benchmarks: HIP events bracket every launch of
  * the named kernel class on the handle's stream when enabled; returns the
  * accumulated milliseconds and launch count. kind: 0 = family update,
- * 1 = flush+wrap, 2 = stabilization (whole get_g_scratch). */
+ * 1 = standalone wrap, 2 = stabilization (whole get_g_scratch).  Inside
+ * pqmc_sweep the wraps are fused into the family passes (kind 0). */
 int pqmc_timing(pqmc_handle *h, int enable, int kind, double *ms, int64_t *launches);
 
 #ifdef __cplusplus

@@ -12,34 +12,9 @@
 
 #include "../../include/pqmc_hip.h"
 #include "pqmc_common.h"
+#include "k_pass.h"
 
 // ---- kernel launchers (k_*.hip) ----
-struct upd_rec {
-  int p;
-  int pad[3];
-  zd relec;
-  zd X[16];
-};
-struct upd_args {
-  model_t md;
-  int f, t, P, chunk;
-  const int *fsites, *fbfi;
-  const zd *fph;
-  const int *fcross;
-  double *x;
-  ekv_t *ekv;
-  const float *u;
-  const zd *G;
-  zd *U, *VT;
-  uint8_t *acc;
-  upd_rec *recs;
-  unsigned long long *sync;
-  unsigned epoch;
-  int *rank_out;
-  double *lncw_acc;
-  unsigned long long *stats;
-  unsigned long long *prof;
-};
 struct fw_args {
   model_t md;
   zd *G;
@@ -59,7 +34,6 @@ struct prop_args {
   int t_first, nslices, backward, fam_only;
 };
 extern "C" {
-hipError_t pqmc_launch_update(const upd_args *a, hipStream_t s);
 hipError_t pqmc_launch_flush_wrap(const fw_args *a, hipStream_t s);
 hipError_t pqmc_launch_propagate(const prop_args *a, hipStream_t s);
 hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, const int *bfi, const int *cross,
@@ -107,7 +81,8 @@ struct ev_pool {
 };
 
 struct pqmc_handle {
-  int L1, L2, Ncell, Ns, ne, Nb, npla, Lt, ngroup, nblock, total_width, P, nbq;
+  int L1, L2, Ncell, Ns, ne, Nb, npla, Lt, ngroup, nblock, total_width, nbq;
+  int nb, nblk, bph, H;  // family pass: bonds per decision block, blocks, bonds per helper, helpers
   model_t md{};
   hipStream_t st = nullptr;
   bool model_set = false, trial_set = false, fields_set = false, g_ready = false;
@@ -117,26 +92,24 @@ struct pqmc_handle {
   double *d_x = nullptr;
   ekv_t *d_ekv = nullptr;
   // Green's function and delayed-update factors
-  zd *d_G = nullptr, *d_Gn = nullptr, *d_U = nullptr, *d_VT = nullptr;
+  zd *d_G = nullptr, *d_Gn = nullptr, *d_scr = nullptr;
   // B-string
   zd *d_Q = nullptr, *d_D = nullptr, *d_R = nullptr;
   zd *d_slQ = nullptr, *d_slD = nullptr, *d_slR = nullptr;
   zd *d_W = nullptr, *d_L = nullptr, *d_M = nullptr, *d_QM = nullptr, *d_Z = nullptr;
   qr_ws qa, qm;
-  // update chain state
+  // family-pass state
   float *d_u = nullptr;
   size_t u_cap = 0;
   uint8_t *d_acc = nullptr;
-  upd_rec *d_recs = nullptr;
-  unsigned long long *d_sync = nullptr;
-  unsigned epoch = 0;
-  int *d_rank = nullptr;
+  zd *d_slot = nullptr;
+  ps_rec *d_rec = nullptr;
+  unsigned long long *d_psync = nullptr;
   double *d_lncw_acc = nullptr;
   zd *d_lncw = nullptr;
   unsigned long long *d_stats = nullptr, *d_err = nullptr;
   zd *d_meas = nullptr, *d_kphase = nullptr;
-  int pending = -1;  // family whose flush is pending
-  unsigned long long *d_prof = nullptr;  // update-chain phase timers (debug)
+  unsigned long long *d_prof = nullptr;  // family-pass phase timers (debug)
   double err_fast_max = 1e-6;
   // timing
   bool timing[3] = {false, false, false};
@@ -210,12 +183,21 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   h->total_width = 3 + 19 * 2 * h->Ncell;
   if (h->ne > h->Ns) { delete h; return fail(PQMC_EINVAL, "ne > Ns"); }
   if (h->Ns > 1024) { delete h; return fail(PQMC_EINVAL, "Ns > 1024 not supported"); }
-  int P = cfg->update_groups;
-  // about 6 bonds per workgroup: per-accept row work is latency bound, so
-  // more workgroups sharing the rows win until handoffs dominate
-  if (P <= 0) P = std::min(16, std::max(1, (h->npla + 5) / 6));
-  while ((h->npla + P - 1) / P > 128) ++P;
-  h->P = P;
+  if (cfg->block_bonds < 0 || cfg->block_bonds > PS_NB) { delete h; return fail(PQMC_EINVAL, "block_bonds out of range"); }
+  h->nb = cfg->block_bonds > 0 ? cfg->block_bonds : PS_NB;
+  h->nblk = (h->npla + h->nb - 1) / h->nb;
+  {
+    // every workgroup of a family pass must be resident at once: one leader
+    // plus ceil(npla / bph) helpers
+    h->bph = 0;
+    for (int bph = 1; bph <= 2 && !h->bph; ++bph) {
+      int maxg = 0;
+      if (pqmc_pass_max_groups(h->Ns, bph, &maxg) != hipSuccess) { delete h; return fail(PQMC_EHIP, "occupancy query failed"); }
+      if ((h->npla + bph - 1) / bph + 1 <= maxg) h->bph = bph;
+    }
+    if (!h->bph) { delete h; return fail(PQMC_EINVAL, "lattice too large for a resident family pass"); }
+    h->H = (h->npla + h->bph - 1) / h->bph;
+  }
   h->nbq = (h->Ns <= 512) ? 16 : 8;
   const int Ns = h->Ns, ne = h->ne, npla = h->npla;
   h->md.Ns = Ns; h->md.ne = ne; h->md.npla = npla; h->md.Nb = h->Nb; h->md.Lt = h->Lt;
@@ -238,8 +220,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_ekv, (size_t)h->Lt * 4 * npla);
   A_(h->d_G, (size_t)Ns * Ns);
   A_(h->d_Gn, (size_t)Ns * Ns);
-  A_(h->d_U, (size_t)Ns * Ns);
-  A_(h->d_VT, (size_t)Ns * Ns);
+  A_(h->d_scr, (size_t)Ns * Ns);
   A_(h->d_Q, (size_t)Ns * ne * h->nblock);
   A_(h->d_D, (size_t)ne * h->nblock);
   A_(h->d_R, (size_t)ne * ne * h->nblock);
@@ -262,9 +243,9 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->qm.beta, ne);
   A_(h->qm.vn2, ne);
   A_(h->d_acc, (size_t)h->Nb * h->Lt);
-  A_(h->d_recs, npla);
-  A_(h->d_sync, 2);
-  A_(h->d_rank, 1);
+  A_(h->d_slot, (size_t)2 * 2 * PS_BS * Ns);
+  A_(h->d_rec, (size_t)h->nblk);
+  A_(h->d_psync, (size_t)h->nblk + 16);
   A_(h->d_lncw_acc, 1);
   A_(h->d_lncw, 1);
   A_(h->d_stats, 4);
@@ -272,7 +253,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_meas, h->total_width);
   A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
 #undef A_
-  if (hipMemsetAsync(h->d_sync, 0, 2 * sizeof(unsigned long long), h->st) != hipSuccess ||
+  if (hipMemsetAsync(h->d_psync, 0, (h->nblk + 16) * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
@@ -306,10 +287,10 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
 extern "C" int pqmc_destroy(pqmc_handle *h) {
   if (!h) return PQMC_OK;
   if (h->st) hipStreamSynchronize(h->st);
-  void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_U,
-                  h->d_VT, h->d_Q, h->d_D, h->d_R, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
+  void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_scr,
+                  h->d_Q, h->d_D, h->d_R, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
-                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_recs, h->d_sync, h->d_rank, h->d_lncw_acc,
+                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -464,36 +445,28 @@ extern "C" int pqmc_set_trial(pqmc_handle *h, const double *slater) {
   return PQMC_OK;
 }
 
-// pending delayed update flushed into G, fused with an optional two-way wrap
-static int flush_wrap(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
-  if (h->pending < 0 && wrap == 0) return PQMC_OK;
+// standalone two-way wrap of G (wrap 1: E G E^-1, 2: E^-1 G E)
+static int wrap_g(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
   fw_args a;
   a.md = h->md;
   a.G = h->d_G;
-  a.U = h->d_U;
-  a.VT = h->d_VT;
-  a.rank = (h->pending >= 0) ? h->d_rank : nullptr;
-  a.pos_ffl = (h->pending >= 0) ? h->d_pos + (size_t)h->pending * h->Ns : nullptr;
-  const int ftile = wrap ? f_wrap : h->pending;
-  a.tsites = h->d_sites + (size_t)ftile * h->npla * 4;
+  a.U = nullptr;
+  a.VT = nullptr;
+  a.rank = nullptr;
+  a.pos_ffl = nullptr;
+  a.tsites = h->d_sites + (size_t)f_wrap * h->npla * 4;
   a.wrap = wrap;
-  a.ekv_w = wrap ? h->d_ekv + ((size_t)(t_wrap - 1) * 4 + f_wrap) * h->npla : nullptr;
-  {
-    span_guard sg(h, 1);
-    HIPCHK(pqmc_launch_flush_wrap(&a, h->st));
-  }
-  h->pending = -1;
+  a.ekv_w = h->d_ekv + ((size_t)(t_wrap - 1) * 4 + f_wrap) * h->npla;
+  span_guard sg(h, 1);
+  HIPCHK(pqmc_launch_flush_wrap(&a, h->st));
   return PQMC_OK;
 }
-
-static int flush(pqmc_handle *h) { return flush_wrap(h, 0, 0, 0); }
 
 extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
   if (!h->model_set || !h->trial_set || !h->fields_set) return fail(PQMC_EINVAL, "model, trial and fields must be set");
   if (start_time < 1 || start_time > h->Lt) return fail(PQMC_EINVAL, "start_time out of range");
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, nb = h->nblock;
-  h->pending = -1;
   HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_x, h->d_bfi, h->d_cross, h->d_ph, h->d_ekv, 1, h->Lt, h->st));
   const size_t qb = sizeof(zd) * Ns * ne;
   HIPCHK(hipMemcpyAsync(QS(h, 1), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
@@ -537,39 +510,36 @@ static int ensure_u(pqmc_handle *h, size_t n) {
   return PQMC_OK;
 }
 
-// launch one family pass; d_u points at its 2*npla draws on the device
-static int launch_update(pqmc_handle *h, int f, int t, const float *d_u, uint8_t *d_acc) {
-  int rc = flush(h);
-  if (rc) return rc;
-  upd_args a;
+// One family pass (update, optionally fused with the family's two-way wrap:
+// wrap 2 = E^-1 G E before the update, 1 = E G E^-1 after it); d_u points at
+// its 2*npla draws on the device.
+static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u, uint8_t *d_acc) {
+  ps_args a;
   a.md = h->md;
   a.f = f;
   a.t = t;
-  a.P = h->P;
-  a.chunk = (h->npla + h->P - 1) / h->P;
+  a.wrap = wrap;
+  a.nb = h->nb;
+  a.nblk = h->nblk;
+  a.bph = h->bph;
+  a.H = h->H;
   a.fsites = h->d_sites + (size_t)f * h->npla * 4;
   a.fbfi = h->d_bfi + (size_t)f * h->npla;
-  a.fph = h->d_ph + (size_t)f * h->npla * 4;
   a.fcross = h->d_cross + (size_t)f * h->npla;
+  a.fph = h->d_ph + (size_t)f * h->npla * 4;
   a.x = h->d_x;
   a.ekv = h->d_ekv;
   a.u = d_u;
   a.G = h->d_G;
-  a.U = h->d_U;
-  a.VT = h->d_VT;
+  a.slot = h->d_slot;
+  a.rec = h->d_rec;
+  a.sync = h->d_psync;
   a.acc = d_acc;
-  a.recs = h->d_recs;
-  a.sync = h->d_sync;
-  a.epoch = ++h->epoch;
-  a.rank_out = h->d_rank;
   a.lncw_acc = h->d_lncw_acc;
   a.stats = h->d_stats;
   a.prof = h->d_prof;
-  {
-    span_guard sg(h, 0);
-    HIPCHK(pqmc_launch_update(&a, h->st));
-  }
-  h->pending = f;
+  span_guard sg(h, 0);
+  HIPCHK(pqmc_launch_pass(&a, h->st));
   return PQMC_OK;
 }
 
@@ -580,7 +550,7 @@ extern "C" int pqmc_update_family(pqmc_handle *h, int pf_id, int time, const flo
   int rc = ensure_u(h, 2 * (size_t)h->npla);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(h->d_u, u, sizeof(float) * 2 * h->npla, hipMemcpyHostToDevice, h->st));
-  rc = launch_update(h, pf_id - 1, time, h->d_u, h->d_acc);
+  rc = launch_pass(h, pf_id - 1, time, 0, h->d_u, h->d_acc);
   if (rc) return rc;
   if (acc) {
     HIPCHK(hipMemcpyAsync(acc, h->d_acc, h->npla, hipMemcpyDeviceToHost, h->st));
@@ -593,7 +563,7 @@ extern "C" int pqmc_evolve_g(pqmc_handle *h, int time, int pf_id, int forward) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
   if (pf_id < 1 || pf_id > 4 || time < 1 || time > h->Lt) return fail(PQMC_EINVAL, "pf_id or time out of range");
-  return flush_wrap(h, forward ? 1 : 2, pf_id - 1, time);
+  return wrap_g(h, forward ? 1 : 2, pf_id - 1, time);
 }
 
 extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left, int two_way, double *mat, int n) {
@@ -604,8 +574,7 @@ extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left,
   const int Ns = h->Ns;
   // ensure the expKV table exists (init_g builds it)
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
-  int rc = flush(h);
-  if (rc) return rc;
+  int rc = PQMC_OK;
   if (two_way) {
     // run the two-way wrap on a scratch copy in d_Gn
     HIPCHK(hipMemcpyAsync(h->d_Gn, mat, sizeof(zd) * Ns * Ns, hipMemcpyHostToDevice, h->st));
@@ -627,8 +596,8 @@ extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left,
     if (left) {
       HIPCHK(hipMemcpyAsync(X, mat, sizeof(zd) * Ns * n, hipMemcpyHostToDevice, h->st));
     } else {
-      HIPCHK(hipMemcpyAsync(h->d_U, mat, sizeof(zd) * Ns * n, hipMemcpyHostToDevice, h->st));
-      HIPCHK(pqmc_conjt(h->d_U, n, n, Ns, X, Ns, h->st));
+      HIPCHK(hipMemcpyAsync(h->d_scr, mat, sizeof(zd) * Ns * n, hipMemcpyHostToDevice, h->st));
+      HIPCHK(pqmc_conjt(h->d_scr, n, n, Ns, X, Ns, h->st));
     }
     prop_args a;
     a.md = h->md;
@@ -644,8 +613,8 @@ extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left,
     if (left) {
       HIPCHK(hipMemcpyAsync(mat, X, sizeof(zd) * Ns * n, hipMemcpyDeviceToHost, h->st));
     } else {
-      HIPCHK(pqmc_conjt(X, Ns, Ns, n, h->d_U, n, h->st));
-      HIPCHK(hipMemcpyAsync(mat, h->d_U, sizeof(zd) * Ns * n, hipMemcpyDeviceToHost, h->st));
+      HIPCHK(pqmc_conjt(X, Ns, Ns, n, h->d_scr, n, h->st));
+      HIPCHK(hipMemcpyAsync(mat, h->d_scr, sizeof(zd) * Ns * n, hipMemcpyDeviceToHost, h->st));
     }
   }
   HIPCHK(hipStreamSynchronize(h->st));
@@ -655,8 +624,7 @@ extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left,
 // get_g_scratch_T0 (src/mod_evolution.f90:113-195)
 static int g_scratch(pqmc_handle *h, int time, int forward) {
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup;
-  int rc = flush(h);
-  if (rc) return rc;
+  int rc = PQMC_OK;
   span_guard sg(h, 2);
   const size_t qb = sizeof(zd) * Ns * ne;
   const zd *lmatH, *rmat;
@@ -714,8 +682,7 @@ extern "C" int pqmc_measure(pqmc_handle *h, int time, double *out) {
   if (!h || !out) return fail(PQMC_EINVAL, "null argument");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
   if (time < 1 || time >= h->Lt) return fail(PQMC_EINVAL, "measurement time must be in [1, Lt-1]");
-  int rc = flush(h);
-  if (rc) return rc;
+  int rc = PQMC_OK;
   HIPCHK(pqmc_launch_measure(&h->md, h->d_G, h->d_x, time, h->d_kphase, h->d_meas, h->st));
   HIPCHK(hipMemcpyAsync(out, h->d_meas, sizeof(zd) * h->total_width, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -738,18 +705,14 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
     if (!forward) {
       time = Lt - tc + 1;
       for (int f = 3; f >= 0; --f) {
-        rc = flush_wrap(h, 2, f, time);
-        if (rc) return rc;
-        rc = launch_update(h, f, time, h->d_u + off, h->d_acc + off / 2);
+        rc = launch_pass(h, f, time, 2, h->d_u + off, h->d_acc + off / 2);
         if (rc) return rc;
         off += 2 * (size_t)npla;
       }
     } else {
       time = tc;
       for (int f = 0; f < 4; ++f) {
-        rc = launch_update(h, f, time, h->d_u + off, h->d_acc + off / 2);
-        if (rc) return rc;
-        rc = flush_wrap(h, 1, f, time);
+        rc = launch_pass(h, f, time, 1, h->d_u + off, h->d_acc + off / 2);
         if (rc) return rc;
         off += 2 * (size_t)npla;
       }
@@ -759,14 +722,10 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
       if (rc) return rc;
     }
     if (measure && time == Lt / 2) {
-      rc = flush(h);
-      if (rc) return rc;
       HIPCHK(pqmc_launch_measure(&h->md, h->d_G, h->d_x, time, h->d_kphase, h->d_meas, h->st));
       measured = true;
     }
   }
-  rc = flush(h);
-  if (rc) return rc;
   if (measured && meas_out)
     HIPCHK(hipMemcpyAsync(meas_out, h->d_meas, sizeof(zd) * h->total_width, hipMemcpyDeviceToHost, h->st));
   if (ntc >= h->ngroup) {
@@ -776,14 +735,13 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   unsigned long long stats[4];
   HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in the update chain");
+  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass");
   return PQMC_OK;
 }
 
 extern "C" int pqmc_get_g(pqmc_handle *h, double *g) {
   if (!h || !g) return fail(PQMC_EINVAL, "null argument");
-  int rc = flush(h);
-  if (rc) return rc;
+  int rc = PQMC_OK;
   HIPCHK(hipMemcpyAsync(g, h->d_G, sizeof(zd) * h->Ns * h->Ns, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   return PQMC_OK;
@@ -791,7 +749,6 @@ extern "C" int pqmc_get_g(pqmc_handle *h, double *g) {
 
 extern "C" int pqmc_set_g(pqmc_handle *h, const double *g) {
   if (!h || !g) return fail(PQMC_EINVAL, "null argument");
-  h->pending = -1;
   HIPCHK(hipMemcpyAsync(h->d_G, g, sizeof(zd) * h->Ns * h->Ns, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   return PQMC_OK;
@@ -842,13 +799,14 @@ extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
   return PQMC_OK;
 }
 
-// Phase timers of the update chain (debug): enable != 0 allocates and
+// Phase timers of the family pass (debug): enable != 0 allocates and
 // zeroes them; out (8*P values, may be NULL) receives microseconds.
 extern "C" int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
-  if (P) *P = h->P;
+  const int G = h->H + 1;
+  if (P) *P = G;
   if (out && h->d_prof) {
-    std::vector<unsigned long long> v(8 * h->P);
+    std::vector<unsigned long long> v(8 * G);
     HIPCHK(hipMemcpyAsync(v.data(), h->d_prof, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     int rate = 0;
@@ -856,8 +814,8 @@ extern "C" int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int3
     for (size_t i = 0; i < v.size(); ++i) out[i] = rate ? 1e3 * (double)v[i] / rate : (double)v[i];
   }
   if (enable) {
-    if (!h->d_prof) HIPCHK(dalloc(&h->d_prof, 8 * h->P));
-    HIPCHK(hipMemsetAsync(h->d_prof, 0, sizeof(unsigned long long) * 8 * h->P, h->st));
+    if (!h->d_prof) HIPCHK(dalloc(&h->d_prof, 8 * G));
+    HIPCHK(hipMemsetAsync(h->d_prof, 0, sizeof(unsigned long long) * 8 * G, h->st));
   } else if (h->d_prof) {
     HIPCHK(hipStreamSynchronize(h->st));
     HIPCHK(hipFree(h->d_prof));

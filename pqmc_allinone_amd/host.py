@@ -73,11 +73,11 @@ class Ran2:
 class Chain:
     """One projector-DQMC Markov chain resident on the GPU."""
 
-    def __init__(self, model: Model, update_groups: int = 0):
+    def __init__(self, model: Model, block_bonds: int = 0):
         self.model = model
         p = model.p
         self._L = _lib.lib()
-        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(update_groups))
+        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(block_bonds))
         h = C.c_void_p()
         _lib.check(self._L.pqmc_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -19,7 +19,7 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def make_pair(L=4, Lt=100, ngroup=10, ne=16, seed=1234, lam=0.64, P=0):
+def make_pair(L=4, Lt=100, ngroup=10, ne=16, seed=1234, lam=0.64, nb=0):
     from pqmc_allinone_amd.host import Chain
     p = ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ngroup, ne=ne, lam=lam)
     m = build_model(p)
@@ -28,7 +28,7 @@ def make_pair(L=4, Lt=100, ngroup=10, ne=16, seed=1234, lam=0.64, P=0):
     orc.set_fields(x)
     orc.set_trial(sl)
     orc.init(1)
-    ch = Chain(m, update_groups=P)
+    ch = Chain(m, block_bonds=nb)
     ch.set_fields(x)
     ch.set_trial(sl)
     ch.init_g(1)
@@ -80,9 +80,9 @@ def test_one_way_evolve_matrix_matches_oracle():
         assert rel(got, ref) < TOL
 
 
-@pytest.mark.parametrize("P", [1, 2])
-def test_update_family_matches_oracle(P):
-    m, orc, ch = make_pair(P=P)
+@pytest.mark.parametrize("nb", [1, 3, 0])
+def test_update_family_matches_oracle(nb):
+    m, orc, ch = make_pair(nb=nb)
     rng = O.Ran2Stream(99)
     for t, f in [(1, 1), (1, 2), (2, 3), (50, 4)]:
         u = rng.draws(2 * m.npla)
@@ -112,10 +112,13 @@ def test_measurement_matches_oracle():
     assert rel(a, b) < TOL
 
 
-@pytest.mark.parametrize("P", [0, 1])
-def test_sweeps_match_oracle(P):
-    """Identical accept/reject sequence and G over alternating sweeps (C1)."""
-    m, orc, ch = make_pair(P=P)
+@pytest.mark.parametrize("nb", [0, 1, 3])
+def test_sweeps_match_oracle(nb):
+    """Identical accept/reject sequence and G over alternating sweeps (C1).
+
+    nb = bonds per decision block: 1 and 3 split the 8 bonds of a family into
+    8 and 3 blocks (the cross-block replay path), 0 = one block."""
+    m, orc, ch = make_pair(nb=nb)
     rng = O.Ran2Stream(1234)
     forward = True
     for sweep in range(4):
@@ -132,3 +135,23 @@ def test_sweeps_match_oracle(P):
     assert c["sync_timeouts"] == 0
     pa, na, _ = orc.counters()
     assert (c["positive_accept"], c["negative_accept"]) == (pa, na)
+
+
+@pytest.mark.parametrize("L,Lt,ntc,nb", [(8, 40, 20, 0), (8, 40, 20, 5), (14, 40, 10, 0)])
+def test_partial_sweeps_match_oracle_larger_lattices(L, Lt, ntc, nb):
+    """8x8 (4 blocks of 8 bonds, or 7 blocks of 5 with a ragged last block) and
+    14x14 (the production size, 13 blocks): identical accepts and G after
+    partial sweeps in both directions, with stabilization every 10 slices."""
+    m, orc, ch = make_pair(L=L, Lt=Lt, ngroup=10, ne=L * L, seed=7, nb=nb)
+    rng = O.Ran2Stream(7)
+    forward = True
+    for sweep in range(2):
+        forward = not forward
+        u = rng.draws(2 * m.Nb * m.p.Lt)
+        ch.sweep(forward, u, ntc=ntc)
+        acc_o, _ = orc.sweep(forward, u, ntc=ntc)
+        n = 4 * m.npla * ntc
+        acc_g = ch.accepts()
+        assert np.array_equal(acc_g[:n], acc_o[:n]), f"sweep {sweep}: first mismatch at {np.argmax(acc_g != acc_o)}"
+        assert rel(ch.g_h, orc.g) < TOL, sweep
+    assert ch.counters()["sync_timeouts"] == 0
