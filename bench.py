@@ -119,36 +119,33 @@ def main():
     dt = time.perf_counter() - t0
     kern = {}
     if timing:
-        for k, name in enumerate(("update_chain", "flush_wrap", "stabilization")):
+        for k, name in enumerate(("family_pass", "wrap", "stabilization")):
             ms, n = ch.timing(k, False)
             kern[name] = {"ms_total": ms, "launches": n, "us_avg": 1e3 * ms / max(n, 1)}
     c1 = ch.counters()
     n_acc = (c1["positive_accept"] - c0["positive_accept"]) / args.steps
     n_prop = n_acc + (c1["negative_accept"] - c0["negative_accept"]) / args.steps
     sps = args.steps / dt
-    Ns, Nb = m.Ns, m.Nb
+    Ns, Nb, npla = m.Ns, m.Nb, m.npla
     F = sweep_flops(Ns, ne, Nb, Lt, ng, n_acc)
     roof = None
     if kern:
         dom = max(kern, key=lambda k: kern[k]["ms_total"])
-        if dom == "flush_wrap":
-            # rank-4k flush (8 Ns^2 r flops, r = 4 accepts of the family) + two-way wrap (64 Ns^2 flops)
-            fl = (8 * Ns ** 2 * 4 * n_acc + Lt * 4 * 64 * Ns ** 2) / max(kern[dom]["launches"] / args.steps, 1)
-            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
-                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
-        elif dom == "update_chain":
-            # delayed-update column work of the chain per family pass: for accept j (rank 4j before it),
-            # 2 x Ns x 4 x 4j complex MACs (8 flops each); accepts per pass = n_acc / (4 Lt)
-            a = n_acc / (4 * Lt)
-            fl = 8 * 2 * Ns * 4 * 4 * (a * (a - 1) / 2)
-            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
-                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
+        per = kern[dom]
+        if dom == "family_pass":
+            # one launch = one family at one slice: the rank-4 update of G for
+            # every accepted bond (8 d Ns^2 flops each, d = 4) plus the fused
+            # two-way wrap (2 * 8 d^2 Ns flops per bond)
+            passes = per["launches"] / args.steps
+            fl = (n_acc / passes) * 8 * 4 * Ns ** 2 + 2 * 8 * 16 * Ns * npla
+        elif dom == "stabilization":
+            fl = 32 * Ns * ne ** 2 + (16 / 3) * ne ** 3 + 8 * Ns ** 2 * ne
         else:
-            fl = (32 * Ns * ne ** 2 + (16 / 3) * ne ** 3 + 8 * Ns ** 2 * ne)
-            roof = {"kernel": dom, "bound": "mfma", "achieved": fl / (kern[dom]["us_avg"] * 1e-6) / 1e12,
-                    "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s"}
-        roof["frac"] = roof["achieved"] / roof["peak"]
-        roof["traffic"] = None
+            fl = 2 * 8 * 16 * Ns * npla
+        achieved = fl / (per["us_avg"] * 1e-6) / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None,
+                "flops_per_launch": fl}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)

@@ -1,0 +1,106 @@
+! Sweep driver over the drop-in subroutines: the loop body of
+! src/Main_PQMC.f90:81-156 with the global update off, called one reference
+! subroutine at a time (update_phonon_field, right_/left_evolve on g_h,
+! get_g_scratch every ngroup slices, begin_measure at ntime/2).
+!
+! usage: pqmc_driver <input file> <output file>
+! Input (stream, little-endian): int32 L1 L2 Lt ngroup ne block_bonds nsweeps
+! seed; int32 sites[4][npla][4], bfi[4][npla], cross[4][npla]; complex128
+! bcph[4][npla][16], kcoe[4], vcoe[4], k12[4][4], v12[4][4], bc[2]; float64
+! dtau M D biased jump_distance hop lam; int32 ncopy tr_weight; float64
+! fields(Nb, Lt); complex128 slater(Ns, ne).
+! Output: int8 accept flags of every update call in call order
+! (nsweeps * Lt * 4 * npla); complex128 G(Ns, Ns) at the end; float64
+! fields(Nb, Lt); complex128 summed measurements (total_width); int32 number
+! of measurements; int64 positive and negative accepts.
+program pqmc_driver
+  use iso_c_binding
+  use pqmc_hip
+  use pqmc_dropin
+  implicit none
+  character(len=1024) :: fin, fout
+  integer :: u_in, u_out
+  integer(c_int32_t) :: hdr(8), ncopy_tr(2)
+  integer :: L1, L2, Lt, ng, ne, nbb, nsweeps, seed, npla_, ns_, nb_
+  integer(c_int32_t), allocatable, target :: sites(:), bfi(:), cross(:)
+  complex(c_double_complex), allocatable, target :: bcph(:), kcoe(:), vcoe(:), k12(:), v12(:), bc(:)
+  real(c_double) :: reals(7)
+  real(c_double), allocatable :: fields(:)
+  complex(c_double_complex), allocatable :: slater(:)
+  type(pqmc_config) :: cfg
+  type(pqmc_model) :: model
+  type(pf_type) :: ppf_list(4)
+  integer(c_int8_t), allocatable :: acc_all(:)
+  integer(c_int64_t) :: cnt(3)
+  real(c_double) :: efm
+  logical :: forward
+  integer :: loop, time_count, time, i_pf, k, pos
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open (newunit=u_in, file=trim(fin), access='stream', form='unformatted', status='old')
+  read (u_in) hdr
+  L1 = hdr(1); L2 = hdr(2); Lt = hdr(3); ng = hdr(4); ne = hdr(5); nbb = hdr(6); nsweeps = hdr(7); seed = hdr(8)
+  ns_ = 2 * L1 * L2
+  nb_ = ns_
+  npla_ = ns_ / 4
+  allocate (sites(4 * npla_ * 4), bfi(4 * npla_), cross(4 * npla_), bcph(4 * npla_ * 16), kcoe(4), vcoe(4), &
+            k12(16), v12(16), bc(2), fields(nb_ * Lt), slater(ns_ * ne))
+  read (u_in) sites, bfi, cross, bcph, kcoe, vcoe, k12, v12, bc, reals, ncopy_tr, fields, slater
+  close (u_in)
+
+  cfg = pqmc_config(L1, L2, Lt, ng, ne, nbb)
+  model%sites = c_loc(sites); model%bfi = c_loc(bfi); model%cross = c_loc(cross); model%bcph = c_loc(bcph)
+  model%kcoe = c_loc(kcoe); model%vcoe = c_loc(vcoe); model%k12 = c_loc(k12); model%v12 = c_loc(v12)
+  model%bc = c_loc(bc)
+  model%dtau = reals(1); model%M = reals(2); model%D = reals(3); model%biased = reals(4)
+  model%jump_distance = reals(5); model%hop = reals(6); model%lam = reals(7)
+  model%ncopy = ncopy_tr(1); model%tr_weight = ncopy_tr(2)
+  call pqmc_open(cfg, model, fields, slater, seed)
+  do k = 1, 4
+    ppf_list(k)%id = k
+  end do
+  call init_g(1)
+
+  allocate (acc_all(nsweeps * Lt * 4 * n_pla))
+  pos = 0
+  forward = .true.
+  do loop = 1, nsweeps
+    forward = .not. forward  ! the first loop is going backwards
+    do time_count = 1, ntime
+      if (.not. forward) then
+        time = ntime - time_count + 1
+        do i_pf = 4, 1, -1
+          call right_evolve(time, g_h, ns, i_pf, two_way=.true.)
+          call update_phonon_field(ppf_list(i_pf), time)
+          acc_all(pos + 1:pos + n_pla) = last_accepts
+          pos = pos + n_pla
+        end do
+      else
+        time = time_count
+        do i_pf = 1, 4
+          call update_phonon_field(ppf_list(i_pf), time)
+          acc_all(pos + 1:pos + n_pla) = last_accepts
+          pos = pos + n_pla
+          call left_evolve(time, g_h, ns, i_pf, two_way=.true.)
+        end do
+      end if
+      if (mod(time_count, ngroup) == 0) then
+        if (forward) then
+          call get_g_scratch(time + 1, forward, loop)
+        else
+          call get_g_scratch(time, forward, loop)
+        end if
+      end if
+      if (time == ntime / 2) call begin_measure(forward, time)
+    end do
+  end do
+
+  call sync_g_h()
+  call pqmc_check(pqmc_get_fields(chain, fields), 'pqmc_get_fields')
+  call pqmc_check(pqmc_get_counters(chain, cnt, efm), 'pqmc_get_counters')
+  open (newunit=u_out, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write (u_out) acc_all, g_h, fields, meas_sum, int(n_meas, c_int32_t), cnt(1), cnt(2)
+  close (u_out)
+  call pqmc_close()
+end program pqmc_driver

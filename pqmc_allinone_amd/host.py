@@ -23,6 +23,7 @@ reference would ``stop``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -238,3 +239,43 @@ class Measurements:
             self.bin_means.append(np.mean(np.real(np.array(self.cur)), axis=0))
             self.cur = []
         return data
+
+
+# ---- Fortran host driver (fortran/pqmc_driver) file formats ---------------
+
+FORTRAN_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "pqmc_driver")
+
+
+def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.ndarray, nsweeps: int, seed: int,
+                       block_bonds: int = 0):
+    """Input file of fortran/pqmc_driver (layout in its header comment)."""
+    p = model.p
+    with open(path, "wb") as f:
+        np.array([p.L1, p.L2, p.Lt, p.ngroup, p.ne, block_bonds, nsweeps, seed], np.int32).tofile(f)
+        np.ascontiguousarray(model.sites, np.int32).tofile(f)
+        np.ascontiguousarray(model.bfi, np.int32).tofile(f)
+        np.ascontiguousarray(model.cross, np.int32).tofile(f)
+        np.ascontiguousarray(np.transpose(model.bcph, (0, 1, 3, 2)), np.complex128).tofile(f)
+        np.ascontiguousarray(model.kcoe, np.complex128).tofile(f)
+        np.ascontiguousarray(model.vcoe, np.complex128).tofile(f)
+        np.ascontiguousarray(np.transpose(model.k12, (0, 2, 1)), np.complex128).tofile(f)
+        np.ascontiguousarray(np.transpose(model.v12, (0, 2, 1)), np.complex128).tofile(f)
+        np.ascontiguousarray(model.bc, np.complex128).tofile(f)
+        np.array([p.dtau, model.M, model.D, p.biased, model.jump_distance, p.hop, p.lam], np.float64).tofile(f)
+        np.array([p.ncopy, int(bool(p.tr_weight))], np.int32).tofile(f)
+        np.ascontiguousarray(fields, np.float64).tofile(f)
+        np.asfortranarray(slater, np.complex128).ravel(order="F").tofile(f)
+
+
+def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
+    p = model.p
+    n_acc = nsweeps * p.Lt * 4 * model.npla
+    with open(path, "rb") as f:
+        acc = np.fromfile(f, np.int8, n_acc).astype(np.uint8)
+        g = np.fromfile(f, np.complex128, model.Ns * model.Ns).reshape((model.Ns, model.Ns), order="F")
+        fields = np.fromfile(f, np.float64, model.Nb * p.Lt).reshape((p.Lt, model.Nb))
+        meas = np.fromfile(f, np.complex128, model.total_width)
+        n_meas = int(np.fromfile(f, np.int32, 1)[0])
+        cnt = np.fromfile(f, np.int64, 2)
+    return {"accepts": acc, "g": g, "fields": fields, "meas_sum": meas, "n_meas": n_meas,
+            "positive_accept": int(cnt[0]), "negative_accept": int(cnt[1])}
