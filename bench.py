@@ -73,6 +73,20 @@ def cpu_baseline(L, Lt, ng, ne, seed, target_s):
                       f"({dt:.1f} s), scaled by slices"}
 
 
+def copy_calibration(mib):
+    """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
+    one dispatch should each read mib * 1024 KiB; the ratio scales the counters
+    of the path's kernels to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.randn(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_kib": mib * 1024, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +97,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
+                         "the known-size reference for the HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs with no "
               "exchange during sweeps, so multi-GPU throughput is N x the one-GPU value", file=sys.stderr)

@@ -8,15 +8,14 @@
 #define PS_BS (4 * PS_NB)   // sites per block
 #define PS_WAIT_TICKS 50000000ull  // 0.5 s of the 100 MHz wall clock
 
+// What a helper needs of a decided block: the a accepted bonds and the
+// 4a x 4a matrix B with  G_after[:, q] = G_before[:, q] + G_before[:, S_A] B w[q],
+// w[q] = (I - G_before)[S_A, q].  The accept count itself travels in the
+// block's flag word.
 struct ps_rec {
-  int a;            // accepted bonds in the block
-  int li[PS_NB];    // their block-local indices, ascending
-  int pad[7];
-  zd X[PS_NB][16];  // X_m, column-major
-  // for accepted m and a later block-local bond j:
-  //   ub[m][j][r*4 + c] = u_m[S_j(r)][c],  vb[m][j][c*4 + s] = v_m[c][S_j(s)]
-  zd ub[PS_NB][PS_NB][16];
-  zd vb[PS_NB][PS_NB][16];
+  int li[PS_NB];          // block-local indices of the accepted bonds, ascending
+  int pad[8];
+  zd B[PS_BS * PS_BS];    // row-major with stride 4a
 };
 
 struct ps_args {
@@ -30,14 +29,24 @@ struct ps_args {
   ekv_t *ekv;         // entry ((t-1)*4 + f)*npla + b
   const float *u;     // 2*npla draws: (ran_sysm source, rands) per bond
   zd *G;              // Ns x Ns, column-major, in/out
-  zd *slot;           // [2][2][PS_BS][Ns] columns of two blocks after a step
+  zd *slot;           // [nblk + 1][2][PS_BS][lds] columns of blocks k+1, k+2 after step k = -1..nblk-1
+  int lds;            // column stride of the slot: Ns rounded up to whole 128-byte lines
   ps_rec *rec;        // [nblk]
-  unsigned long long *sync;  // [0] blocks published, [1] done, [2] abort, [8 + k + 1] arrivals after step k
+  unsigned long long *sync;  // [1] done, [2] abort, [8 + k + 1] arrivals after step k (k = -1..nblk-1),
+                             // [9 + nblk + k] block k published: accepts + 1
   uint8_t *acc;       // [npla]
   double *lncw_acc;
   unsigned long long *stats;  // [0] accepted, [1] rejected, [2] wait timeouts
   unsigned long long *prof;   // optional [gridDim][8] wall-clock ticks
 };
+
+// column stride and size of the slot
+static inline int pqmc_pass_slot_ld(int Ns) { return (Ns + 7) & ~7; }
+static inline size_t pqmc_pass_slot_elems(int Ns, int nblk) {
+  return (size_t)(nblk + 1) * 2 * PS_BS * pqmc_pass_slot_ld(Ns);
+}
+// words of the sync array for nblk decision blocks
+static inline size_t pqmc_pass_sync_words(int nblk) { return 16 + 2 * (size_t)nblk; }
 
 extern "C" {
 size_t pqmc_pass_lds(int Ns, int bph);

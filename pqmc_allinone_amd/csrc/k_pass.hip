@@ -7,28 +7,30 @@
 // bond i, applies the rank-4 update
 //     G <- G + u_i v_i,   u_i = G[:, S_i] X_i,   v_i = (I - G)[S_i, :],
 //     X_i = Delta_i (I + G[S_i, S_i] Delta_i)^-1.
-// Here the bonds are cut into decision blocks of nb consecutive bonds and
-// the work is split between one leader workgroup and H helper workgroups:
+// The bonds of one family act on disjoint sites, so the updates of the a
+// accepted bonds of any stretch of bonds fold into one update of rank 4a:
+//     G_after[:, q] = G[:, q] + G[:, S_A] B w[q],   w[q] = (I - G)[S_A, q],
+// where G is taken before the stretch and the 4a x 4a matrix B = A T^-1
+// collects the sequence:
+//     A[:, m] = (sel_m + sum_{m'<m} A[:, m'] v_{m'}[S_m]) X_m   (block upper),
+//     T[m][m'] = u_{m'}[S_m] for m' < m, identity blocks on the diagonal.
+// The bonds are cut into decision blocks of nb consecutive bonds; one leader
+// workgroup decides them and H helper workgroups own the columns of G:
 //
-//  * Helper h owns the columns of bph bonds and keeps them in LDS for the
-//    whole pass.  After block k is decided it applies the block's accepts
-//    to its columns in one step: with S_A the sites of the a accepted bonds,
-//        G[:, q] += G_k[:, S_A] (A v[q]),
-//    where G_k is G before block k, v[q] stacks v_m[q] (m = 1..a) and the
-//    4a x 4a block-upper-triangular A folds the in-block sequence:
-//        A[:, m] = (sel_m + sum_{m'<m} A[:, m'] v_{m'}[S_m]) X_m,
-//        v_m[q]  = e_q(S_m) - G_k[S_m, q] - sum_{m'<m} u_{m'}[S_m] v_{m'}[q].
-//    The pivot columns G_k[:, S_A] are read from a slot that the owners of
-//    block k's columns filled after step k-1.
-//  * The leader decides the bonds of block k in one wave from an LDS copy
+//  * Leader, wave 0: decides the bonds of block k from an LDS copy
 //    W = G[S_k, S_k] that it keeps current with the rank-4 updates (only the
-//    rows and columns of the block's later bonds change).  It publishes the
-//    accepted bonds with X_m and the 4x4 blocks u_m[S_j], v_m[S_j] of every
-//    later bond j, which is all a helper needs.  Before block k+1 it needs
-//    G[S_{k+1}, S_{k+1}] after block k: it takes G[S_{k+1}, S_k],
-//    G[S_k, S_{k+1}] and G[S_{k+1}, S_{k+1}] after block k-1 from the slot
-//    of step k-1 (filled while it decided block k) and replays block k's
-//    accepts on them.  The helpers therefore lag the leader by one block.
+//    rows and columns of the block's later bonds change).
+//  * Leader, wave 1: builds B of block k accept by accept while wave 0 goes
+//    on, and publishes it with the accepted indices.
+//  * Leader, waves 2-7: prepare the proposals of block k+1 and fetch
+//    G[S_{k+1}, S_k], G[S_k, S_{k+1}], G[S_{k+1}, S_{k+1}] after block k-1.
+//    After block k the whole workgroup folds block k into the last tile,
+//        G[S_{k+1}, S_{k+1}] -= (G[S_{k+1}, S_A] B) G[S_A, S_{k+1}],
+//    which is the next window.  The helpers therefore lag by one block.
+//  * Helper h keeps the columns of bph bonds in LDS for the whole pass and
+//    applies block k to them in one step, G[:, q] += G[:, S_A] (B w[q]), with
+//    the pivot columns G[:, S_A] read from a slot that the owners of block
+//    k's columns filled after step k-1.
 //
 // Accept test: the reference rejects when prob < u with prob =
 // min(1, exp(min(10, w))), w = dS + ncopy*(1+TR)*log|det(I + G_SS Delta)|
@@ -47,12 +49,17 @@ struct ps_prop {
 };
 
 // Data exchanged between the workgroups of a pass (records, slot columns,
-// updated exponentials) is written and read only with device-scope
-// (sc1) accesses, which are coherent across the XCDs' L2 caches.  A
-// publisher therefore needs no L2 write-back: it waits for its own stores
-// to complete (vmcnt 0) and then raises a flag; a reader that saw the flag
-// issues its coherent loads afterwards.  Agent-scope fences, which write
-// back and invalidate the whole L2 of the XCD, are avoided on purpose.
+// updated exponentials) is written only with device-scope (sc1) stores,
+// which go through to memory, and a publisher waits for its own stores to
+// complete (vmcnt 0) before it raises a flag.  Agent-scope fences, which
+// write back and invalidate the whole L2 of the XCD, are avoided on purpose.
+// Readers use plain, L2-cached loads for records and slot columns: every
+// such address is written exactly once per pass (one slot per step) and is
+// read only after its flag, so no cache can hold an older copy of it from
+// this pass, and the caches are invalidated at kernel start.  The H helpers
+// that read one block's columns then share one copy per XCD L2 instead of
+// all fetching the same lines from memory.  Flags and the exponentials,
+// which are read before and after they change, use coherent loads.
 static __device__ __forceinline__ zd ld_coh(const zd *p) {
   const double *d = reinterpret_cast<const double *>(p);
   return zmk(__hip_atomic_load(d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
@@ -97,10 +104,11 @@ static __device__ __forceinline__ void ps_st(unsigned long long *p, unsigned lon
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Single-lane wait until *p >= v (and *q >= w when q is set).  Bounded by
-// wall-clock time and by the abort word.
-static __device__ bool ps_wait(const ps_args &A, unsigned long long *p, unsigned long long v,
-                               unsigned long long *q, unsigned long long w) {
+// Single-lane wait until *p >= v (and *q >= w when q is set); returns the
+// last value read from p, or 0 when the wait failed.  Bounded by wall-clock
+// time and by the abort word.
+static __device__ unsigned long long ps_wait(const ps_args &A, unsigned long long *p, unsigned long long v,
+                                             unsigned long long *q, unsigned long long w) {
   // one round trip per poll: the flag words are loaded together; the abort
   // word and the clock are looked at every 32 polls
   const unsigned long long t0 = wall_clock64();
@@ -110,14 +118,14 @@ static __device__ bool ps_wait(const ps_args &A, unsigned long long *p, unsigned
     const unsigned long long a = ps_ld(p), b = ps_ld(q2);
     if (a >= v && b >= w2) {
       ps_acquire();
-      return true;
+      return a;
     }
     if ((it & 31) == 0) {
-      if (ps_ld(&A.sync[2]) != 0) return false;
+      if (ps_ld(&A.sync[2]) != 0) return 0;
       if (wall_clock64() - t0 > PS_WAIT_TICKS) {
         __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&A.stats[2], 1ull);
-        return false;
+        return 0;
       }
     }
     __builtin_amdgcn_s_sleep(1);
@@ -125,9 +133,12 @@ static __device__ bool ps_wait(const ps_args &A, unsigned long long *p, unsigned
 }
 
 static __device__ __forceinline__ unsigned long long *ps_arr(const ps_args &A, int k) { return &A.sync[8 + k + 1]; }
-static __device__ __forceinline__ int ps_sidx(int k) { return (k + 1) & 1; }  // slot written after step k
-static __device__ __forceinline__ zd *ps_scol(const ps_args &A, int s, int mb, int cl) {
-  return A.slot + ((size_t)((s * 2 + mb) * PS_BS + cl)) * A.md.Ns;
+static __device__ __forceinline__ unsigned long long *ps_flag(const ps_args &A, int k) {
+  return &A.sync[9 + A.nblk + k];
+}
+// column cl of block kstep+1+mb in the slot written after step kstep
+static __device__ __forceinline__ zd *ps_scol(const ps_args &A, int kstep, int mb, int cl) {
+  return A.slot + ((size_t)(((kstep + 1) * 2 + mb) * PS_BS + cl)) * A.lds;
 }
 static __device__ __forceinline__ int ps_nsites(const ps_args &A, int k) {
   return 4 * min(A.nb, A.md.npla - k * A.nb);
@@ -139,11 +150,16 @@ static __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-static __device__ __forceinline__ zd shfl_z(zd v, int src) {
-  return zmk(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+// sum over the lanes l, l+4, l+8, l+12 of each 16-lane row
+static __device__ __forceinline__ double row4_sum(double v) {
+  v += dpp_move<0x124, 0xF>(v);  // row_ror:4
+  v += dpp_move<0x128, 0xF>(v);  // row_ror:8
+  return v;
 }
-static __device__ __forceinline__ zd shfl_xor_z(zd v, int m) {
-  return zmk(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+static __device__ __forceinline__ double lane0(double v) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 0);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 0);
+  return __hiloint2double(hi, lo);
 }
 
 // modi of src/mod_matrixlib.f90:66-77
@@ -166,14 +182,23 @@ static __device__ __forceinline__ zd minor3(const zd *M, int ri, int cj) {
   return t;
 }
 
-#define PROF_T0() unsigned long long _pt = A.prof ? wall_clock64() : 0ull
-#define PROF_ACC(k)                                                         \
-  do {                                                                      \
-    if (A.prof && threadIdx.x == 0) {                                       \
-      const unsigned long long _now = wall_clock64();                       \
-      A.prof[blockIdx.x * 8 + (k)] += _now - _pt;                           \
-      _pt = _now;                                                           \
-    }                                                                       \
+// Optional phase timers (wall clock), kept in registers and added to
+// A.prof[blockIdx.x * 8 + phase] once at the end.
+#define PROF_T0()                                                \
+  unsigned long long _pt = A.prof ? wall_clock64() : 0ull;       \
+  unsigned long long _pa[8] = {0, 0, 0, 0, 0, 0, 0, 0}
+#define PROF_ACC(k)                                              \
+  do {                                                           \
+    if (A.prof) {                                                \
+      const unsigned long long _now = wall_clock64();            \
+      _pa[k] += _now - _pt;                                      \
+      _pt = _now;                                                \
+    }                                                            \
+  } while (0)
+#define PROF_FLUSH()                                                                      \
+  do {                                                                                    \
+    if (A.prof && threadIdx.x == 0)                                                       \
+      for (int _k = 0; _k < 8; ++_k) A.prof[blockIdx.x * 8 + _k] += _pa[_k];              \
   } while (0)
 
 // Proposal of bond p (family order): jump, bosonic action change
@@ -216,33 +241,112 @@ static __device__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
     }
 }
 
-// Gather a tile G[rows of block rb, cols of block cb] from the slot holding
-// blocks (base, base+1) after a step.  T is row-major with stride PS_BS.
-static __device__ void ps_load_tile(const ps_args &A, int s, int base, int rb, int cb, zd *T, int t0, int nt) {
+// Gather a tile G[rows of block rb, cols of block cb] from the slot written
+// after step kstep (blocks kstep+1, kstep+2).  T is row-major with stride PS_BS.
+static __device__ void ps_load_tile(const ps_args &A, int kstep, int rb, int cb, zd *T, int t0, int nt) {
   const int nr = ps_nsites(A, rb), nc = ps_nsites(A, cb);
   const int *rs = A.fsites + 4 * A.nb * rb;
   for (int e = t0; e < PS_BS * PS_BS; e += nt) {
     const int r = e / PS_BS, c = e % PS_BS;
     zd v = zmk(0.0, 0.0);
-    if (r < nr && c < nc) v = ld_coh(ps_scol(A, s, cb - base, c) + rs[r]);
+    if (r < nr && c < nc) v = ps_scol(A, kstep, cb - (kstep + 1), c)[rs[r]];
     T[e] = v;
   }
 }
 
 // --------------------------------------------------------------------------
-// Leader: decisions, records, replay.
+// Leader: decisions (wave 0), B (wave 1), next block (waves 2-7), fold.
 // --------------------------------------------------------------------------
 struct ps_leader_lds {
-  zd buf[4][PS_BS * PS_BS];     // W, T10, T01, T11 (rotating)
+  zd buf[4][PS_BS * PS_BS];     // W, T10, T01, T11 (W and T11 rotate)
   zd sU[PS_NB][PS_BS * 4];      // u_m at the block's rows (row*4 + comp)
   zd sV[PS_NB][4 * PS_BS];      // v_m at the block's cols (comp*BS + col)
-  zd sUN[PS_BS * 4], sVN[4 * PS_BS];
   zd sX[PS_NB][16];
-  zd sM[16];
+  zd sA[PS_BS * PS_BS];         // A (row-major, stride PS_BS); the fold's G[S_{k+1}, S_A] B afterwards
+  zd sTi[PS_BS * PS_BS];        // T^-1 (row-major, stride PS_BS)
+  zd sB[PS_BS * PS_BS];         // B = A T^-1 (row-major, stride PS_BS)
+  zd sTt[PS_BS * 4];
+  zd sM[16], sCof[16];
+  zd dets[PS_NB];
   ps_prop prop[2][PS_NB];
   int li[PS_NB];
-  int nacc, abort;
+  int npub, done, nacc, abort;
 };
+
+// Wave 1: fold accept m of the current block into A, T^-1 and B.
+static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
+  const int n = 4 * (m + 1);
+  const int jm = L.li[m];
+  zd *__restrict__ sA = L.sA;
+  zd *__restrict__ sTi = L.sTi;
+  zd *__restrict__ sB = L.sB;
+  zd *__restrict__ sTt = L.sTt;
+  // T(g, kk) = sel + sum_{m'<m} A[g][4m'+d] v_{m'}[d][S_m(kk)]  (A[g][4m'+d] = 0 unless g/4 <= m')
+  // T^-1[4m + r][g] = sel - sum_{m'<m} u_{m'}[S_m(r)][d] T^-1[4m'+d][g]  (zero unless g/4 <= m')
+  for (int e = lane; e < 8 * n; e += 64) {
+    if (e < 4 * n) {
+      const int g = e >> 2, kk = e & 3;
+      zd t = zmk(g == 4 * m + kk ? 1.0 : 0.0, 0.0);
+      for (int mp = g >> 2; mp < m; ++mp) {
+        zd av[4], vv[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          av[d] = sA[g * PS_BS + 4 * mp + d];
+          vv[d] = L.sV[mp][d * PS_BS + 4 * jm + kk];
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) t = zadd(t, zmul(av[d], vv[d]));
+      }
+      sTt[g * 4 + kk] = t;
+    } else {
+      const int e2 = e - 4 * n;
+      const int r = e2 / n, g = e2 % n;
+      zd t = zmk(g == 4 * m + r ? 1.0 : 0.0, 0.0);
+      for (int mp = g >> 2; mp < m; ++mp) {
+        zd uv[4], tv[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          uv[d] = L.sU[mp][(4 * jm + r) * 4 + d];
+          tv[d] = sTi[(4 * mp + d) * PS_BS + g];
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) t = zsub(t, zmul(uv[d], tv[d]));
+      }
+      sTi[(4 * m + r) * PS_BS + g] = t;
+    }
+  }
+  wave_sync();
+  // A[:, block m] = T X_m
+  for (int e = lane; e < 4 * n; e += 64) {
+    const int g = e >> 2, c = e & 3;
+    zd tv[4], xv[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      tv[kk] = sTt[g * 4 + kk];
+      xv[kk] = L.sX[m][kk + 4 * c];
+    }
+    zd s = zmk(0.0, 0.0);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) s = zadd(s, zmul(tv[kk], xv[kk]));
+    sA[g * PS_BS + 4 * m + c] = s;
+  }
+  wave_sync();
+  // B += A[:, block m] T^-1[block m, :]
+  for (int e = lane; e < n * n; e += 64) {
+    const int g = e / n, h = e % n;
+    zd av[4], tv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      av[c] = sA[g * PS_BS + 4 * m + c];
+      tv[c] = sTi[(4 * m + c) * PS_BS + h];
+    }
+    zd s = sB[g * PS_BS + h];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s = zadd(s, zmul(av[c], tv[c]));
+    sB[g * PS_BS + h] = s;
+  }
+  wave_sync();
+}
 
 static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -256,11 +360,13 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
   if (tid < min(nb, npla)) ps_propose(A, tid, L.prop[0][tid]);
   if (tid == 0) {
     L.abort = 0;
+    L.npub = 0;
+    L.done = 0;
     L.nacc = ps_wait(A, ps_arr(A, -1), (unsigned long long)A.H, nullptr, 0) ? 1 : 0;
   }
   __syncthreads();
   ok = L.nacc != 0;
-  if (ok) ps_load_tile(A, ps_sidx(-1), 0, 0, 0, L.buf[iW], tid, PS_THREADS);
+  if (ok) ps_load_tile(A, -1, 0, 0, L.buf[iW], tid, PS_THREADS);
   __syncthreads();
   PROF_ACC(0);
   int total_acc = 0, total_rej = 0;
@@ -275,58 +381,60 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       const ps_prop *pp = L.prop[k & 1];
       int m = 0;
       for (int j = 0; j < nbk; ++j) {
-        // M = I + G_SS Delta (src/mod_update.f90:97-100), one entry per lane
+        // M = I + G_SS Delta (src/mod_update.f90:97-100), lane a + 4c holds M(a, c)
+        const int ra = lane & 3, rc = (lane >> 2) & 3;
         if (lane < 16) {
-          const int a = lane & 3, c = lane >> 2;
-          zd mv = zmk(a == c ? 1.0 : 0.0, 0.0);
+          zd mv = zmk(ra == rc ? 1.0 : 0.0, 0.0);
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk) mv = zadd(mv, zmul(W[(4 * j + a) * PS_BS + 4 * j + kk], pp[j].D[kk + 4 * c]));
+          for (int kk = 0; kk < 4; ++kk) mv = zadd(mv, zmul(W[(4 * j + ra) * PS_BS + 4 * j + kk], pp[j].D[kk + 4 * rc]));
           L.sM[lane] = mv;
         }
         wave_sync();
-        zd cof = zmk(0.0, 0.0);
+        // cofactor C(ra, rc); det = sum_c M(0, c) C(0, c) over lanes 0, 4, 8, 12
+        zd cof = zmk(0.0, 0.0), term = zmk(0.0, 0.0);
         if (lane < 16) {
-          const int ri = lane & 3, cj = lane >> 2;
-          cof = minor3(L.sM, ri, cj);
-          if ((ri + cj) & 1) cof = zneg(cof);
+          cof = minor3(L.sM, ra, rc);
+          if ((ra + rc) & 1) cof = zneg(cof);
+          if (ra == 0) term = zmul(L.sM[lane], cof);
         }
-        // det = sum_j M(0,j) C(0,j): lanes 0, 4, 8, 12
-        zd term = (lane < 16 && (lane & 3) == 0) ? zmul(L.sM[lane], cof) : zmk(0.0, 0.0);
-        term = zadd(term, shfl_xor_z(term, 4));
-        term = zadd(term, shfl_xor_z(term, 8));
-        const zd det = shfl_z(term, 0);
-        const bool acc = !((det.x * det.x + det.y * det.y) < pp[j].thr);
+        const double dre = lane0(row4_sum(term.x)), dim = lane0(row4_sum(term.y));
+        const double n2 = dre * dre + dim * dim;
+        const bool acc = !(n2 < pp[j].thr);
         if (lane == 0) A.acc[p0 + j] = (uint8_t)acc;
         PROF_ACC(4);
         if (!acc) {
           ++total_rej;
           continue;
         }
-        // X = Delta M^-1 with M^-1 = adj(M)/det; lane k + 4c holds X(k, c)
-        const int ia = lane & 3, ic = (lane >> 2) & 3;
-        const zd inv = zdiv(shfl_z(cof, ic + 4 * ia), det);
-        zd xv = zmk(0.0, 0.0);
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) xv = zadd(xv, zmul(pp[j].D[ia + 4 * kk], shfl_z(inv, kk + 4 * ic)));
+        // X = Delta M^-1 with M^-1(kk, c) = C(c, kk) / det
+        if (lane < 16) L.sCof[lane] = cof;
+        wave_sync();
         if (lane < 16) {
-          L.sX[m][lane] = xv;
-          st_coh(&A.rec[k].X[m][lane], xv);
+          zd dv[4], cv[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            dv[kk] = pp[j].D[ra + 4 * kk];
+            cv[kk] = L.sCof[rc + 4 * kk];
+          }
+          const double in2 = 1.0 / n2;
+          const zd idet = zmk(dre * in2, -dim * in2);
+          zd xv = zmk(0.0, 0.0);
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) xv = zadd(xv, zmul(dv[kk], cv[kk]));
+          L.sX[m][lane] = zmul(xv, idet);
         }
         if (lane == 0) {
+          const ekv_t en = pp[j].en;
+          const double xn = pp[j].xnew;
+          const int fld = pp[j].fld;
           L.li[m] = j;
-          const int p = p0 + j;
-          A.x[pp[j].fld + tbase] = pp[j].xnew;
-          st_coh_ekv(&A.ekv[((size_t)(A.t - 1) * 4 + A.f) * npla + p], pp[j].en);
-          // ln_cw += log(real(R_elec)) (src/mod_update.f90:142)
-          zd rel = det;
-          if (md.tr_weight) rel = zmul(rel, zconj(rel));
-          zd rp = rel;
-          for (int c = 1; c < md.ncopy; ++c) rp = zmul(rp, rel);
-          atomicAdd(A.lncw_acc, log(rp.x));
+          L.dets[m] = zmk(dre, dim);
+          A.x[fld + tbase] = xn;
+          st_coh_ekv(&A.ekv[((size_t)(A.t - 1) * 4 + A.f) * npla + p0 + j], en);
         }
         wave_sync();
         PROF_ACC(5);
-        // u_m, v_m at the later sites of the block; then the window update
+        // u_m, v_m at the later sites of the block
         const int o = 4 * (j + 1), Ln = nsk - o;
         for (int e = lane; e < Ln * 4; e += 64) {
           const int r = o + (e >> 2), c = e & 3;
@@ -338,20 +446,47 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
           L.sV[m][ca * PS_BS + cc] = zneg(W[(4 * j + ca) * PS_BS + cc]);
         }
         wave_sync();
+        // hand accept m to wave 1
+        if (lane == 0) __hip_atomic_store(&L.npub, m + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         PROF_ACC(6);
-        for (int e = lane; e < Ln * 4; e += 64) {
-          // ub[m][jj][r*4 + c] = u_m[S_jj(r)][c]; vb[m][jj][c*4 + s] = v_m[c][S_jj(s)]
-          const int r = o + (e >> 2), c = e & 3;
-          st_coh(&A.rec[k].ub[m][r >> 2][(r & 3) * 4 + c], L.sU[m][r * 4 + c]);
-          const int ca = e / Ln, cc = o + e % Ln;
-          st_coh(&A.rec[k].vb[m][cc >> 2][ca * 4 + (cc & 3)], L.sV[m][ca * PS_BS + cc]);
-        }
-        for (int e = lane; e < Ln * Ln; e += 64) {
-          const int r = o + e / Ln, c = o + e % Ln;
-          zd s = W[r * PS_BS + c];
+        // W += u_m v_m on the later rows and columns; lane (rr, cl) of an 8 x 8
+        // grid owns rows o + rr + 8i and columns o + cl + 8i2.  Operands are
+        // read into registers before any write (all addresses clamped in range).
+        if (Ln > 0) {
+          zd *__restrict__ Wr = W;
+          const zd *__restrict__ Ur = L.sU[m];
+          const zd *__restrict__ Vr = L.sV[m];
+          const int rr = lane >> 3, cl = lane & 7;
+          const int ni = (Ln + 7) >> 3;
+          zd vv[4][4];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) s = zadd(s, zmul(L.sU[m][r * 4 + a], L.sV[m][a * PS_BS + c]));
-          W[r * PS_BS + c] = s;
+          for (int i2 = 0; i2 < 4; ++i2) {
+            const int c = o + cl + 8 * i2;
+            const bool cv = i2 < ni && c < nsk;
+            const int cc = min(c, PS_BS - 1);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              const zd t = Vr[a * PS_BS + cc];
+              vv[i2][a] = cv ? t : zmk(0.0, 0.0);
+            }
+          }
+          for (int i = 0; i < ni; ++i) {
+            const int r = min(o + rr + 8 * i, PS_BS - 1);
+            const bool rv = o + rr + 8 * i < nsk;
+            zd uu[4], w[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) uu[a] = Ur[r * 4 + a];
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) w[i2] = Wr[r * PS_BS + min(o + cl + 8 * i2, PS_BS - 1)];
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+              const int c = o + cl + 8 * i2;
+              zd s = w[i2];
+#pragma unroll
+              for (int a = 0; a < 4; ++a) s = zadd(s, zmul(uu[a], vv[i2][a]));
+              if (rv && i2 < ni && c < nsk) Wr[r * PS_BS + c] = s;
+            }
+          }
         }
         wave_sync();
         PROF_ACC(7);
@@ -359,26 +494,52 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         ++total_acc;
       }
       if (lane == 0) {
-        st_coh_i(&A.rec[k].a, m);
-        for (int q = 0; q < m; ++q) st_coh_i(&A.rec[k].li[q], L.li[q]);
         L.nacc = m;
+        __hip_atomic_store(&L.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      ps_release();
-      if (lane == 0) ps_st(&A.sync[0], (unsigned long long)(k + 1));
+      ps_release();  // acc flags, fields and exponentials are complete before block k is published
       PROF_ACC(1);
+    } else if (wave == 1) {
+      // ---- B of block k, accept by accept ----
+      for (int e = lane; e < PS_BS * PS_BS; e += 64) L.sB[e] = zmk(0.0, 0.0);
+      wave_sync();
+      int md_ = 0;
+      const unsigned long long t0 = wall_clock64();
+      for (unsigned it = 1;; ++it) {
+        // wave 0 never waits inside a block, so this ends once it is done;
+        // the clock bound only guards against a logic error
+        int np = __hip_atomic_load(&L.npub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (np <= md_) {
+          if (__hip_atomic_load(&L.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            np = __hip_atomic_load(&L.npub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (np <= md_) break;
+          } else {
+            if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
+              if (lane == 0) {
+                __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicAdd(&A.stats[2], 1ull);
+                L.abort = 1;
+              }
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+        }
+        for (; md_ < np; ++md_) ps_fold_accept(L, md_, lane);
+      }
     } else if (has_next) {
-      // ---- waves 1..7: next block's proposals and tiles after step k-1 ----
-      const int t1 = tid - 64, nt1 = PS_THREADS - 64;
+      // ---- waves 2..7: next block's proposals and tiles after step k-1 ----
+      const int t1 = tid - 128, nt1 = PS_THREADS - 128;
       const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
       if (t1 < nb1) ps_propose(A, p1 + t1, L.prop[(k + 1) & 1][t1]);
       bool wok = true;
-      if (lane == 0) wok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0);
+      if (lane == 0) wok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
       wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
       if (wok) {
-        const int s = ps_sidx(k - 1);
-        ps_load_tile(A, s, k, k + 1, k, L.buf[i10], t1, nt1);
-        ps_load_tile(A, s, k, k, k + 1, L.buf[i01], t1, nt1);
-        ps_load_tile(A, s, k, k + 1, k + 1, L.buf[i11], t1, nt1);
+        ps_load_tile(A, k - 1, k + 1, k, L.buf[i10], t1, nt1);
+        ps_load_tile(A, k - 1, k, k + 1, L.buf[i01], t1, nt1);
+        ps_load_tile(A, k - 1, k + 1, k + 1, L.buf[i11], t1, nt1);
       } else if (lane == 0) {
         L.abort = 1;
       }
@@ -387,50 +548,91 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     __syncthreads();
     if (L.abort) { ok = false; break; }
     PROF_ACC(2);
+    const int a = L.nacc, n4 = 4 * a;
+    if (wave == 1) {
+      // ---- publish block k: accepted indices, B, then the flag ----
+      ps_rec *R = A.rec + k;
+      if (lane < a) st_coh_i(&R->li[lane], L.li[lane]);
+      for (int e = lane; e < n4 * n4; e += 64) st_coh(&R->B[e], L.sB[(e / n4) * PS_BS + e % n4]);
+      ps_release();
+      if (lane == 0) ps_st(ps_flag(A, k), (unsigned long long)(a + 1));
+      // ln_cw += log(real(R_elec)) per accept (src/mod_update.f90:142)
+      double lv = 0.0;
+      if (lane < a) {
+        zd rel = L.dets[lane];
+        if (md.tr_weight) rel = zmul(rel, zconj(rel));
+        zd rp = rel;
+        for (int c = 1; c < md.ncopy; ++c) rp = zmul(rp, rel);
+        lv = log(rp.x);
+      }
+      lv = wave_sum(lv);
+      if (lane == 0 && a > 0) atomicAdd(A.lncw_acc, lv);
+    }
     if (!has_next) break;
-    // ---- replay block k's accepts on the tiles of block k+1 ----
-    const int m_acc = L.nacc;
     const int nn = ps_nsites(A, k + 1);
     zd *T10 = L.buf[i10], *T01 = L.buf[i01], *T11 = L.buf[i11];
-    for (int mm = 0; mm < m_acc; ++mm) {
-      const int j = L.li[mm];
-      const int o = 4 * (j + 1), Ln = nsk - o;
-      for (int e = tid; e < nn * 4; e += PS_THREADS) {
-        const int r = e >> 2, c = e & 3;
-        zd s = zmk(0.0, 0.0);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) s = zadd(s, zmul(T10[r * PS_BS + 4 * j + a], L.sX[mm][a + 4 * c]));
-        L.sUN[r * 4 + c] = s;
-        const int ca = e / nn, cc = e % nn;
-        L.sVN[ca * PS_BS + cc] = zneg(T01[(4 * j + ca) * PS_BS + cc]);
-      }
-      __syncthreads();
-      const int n10 = nn * Ln, n01 = Ln * nn, n11 = nn * nn;
-      for (int e = tid; e < n10 + n01 + n11; e += PS_THREADS) {
-        if (e < n10) {
-          const int r = e / Ln, c = o + e % Ln;
-          zd s = T10[r * PS_BS + c];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) s = zadd(s, zmul(L.sUN[r * 4 + a], L.sV[mm][a * PS_BS + c]));
-          T10[r * PS_BS + c] = s;
-        } else if (e < n10 + n01) {
-          const int e2 = e - n10;
-          const int r = o + e2 / nn, c = e2 % nn;
-          zd s = T01[r * PS_BS + c];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) s = zadd(s, zmul(L.sU[mm][r * 4 + a], L.sVN[a * PS_BS + c]));
-          T01[r * PS_BS + c] = s;
-        } else {
-          const int e2 = e - n10 - n01;
-          const int r = e2 / nn, c = e2 % nn;
-          zd s = T11[r * PS_BS + c];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) s = zadd(s, zmul(L.sUN[r * 4 + a], L.sVN[a * PS_BS + c]));
-          T11[r * PS_BS + c] = s;
+    if (a > 0) {
+      // ---- fold block k into the next window: T11 -= (T10[:, S_A] B) T01[S_A, :] ----
+      // gather the accepted columns of T10 and rows of T01 (wave 1 is publishing)
+      zd *T10c = L.sTi, *T01c = L.buf[iW], *C = L.sA;
+      if (wave != 1) {
+        const int t2 = wave == 0 ? tid : tid - 64;
+        for (int e = t2; e < 2 * nn * n4; e += PS_THREADS - 64) {
+          const int e2 = e < nn * n4 ? e : e - nn * n4;
+          const int x = e2 / n4, g = e2 % n4;
+          const int cg = 4 * L.li[g >> 2] + (g & 3);
+          if (e < nn * n4)
+            T10c[x * PS_BS + g] = T10[x * PS_BS + cg];
+          else
+            T01c[g * PS_BS + x] = T01[cg * PS_BS + x];
         }
       }
       __syncthreads();
+      if (wave != 1) {
+        const int t2 = wave == 0 ? tid : tid - 64;
+        const zd *__restrict__ Tc = T10c;
+        const zd *__restrict__ Bm = L.sB;
+        for (int e = t2; e < nn * n4; e += PS_THREADS - 64) {
+          const int r = e / n4, g = e % n4;
+          zd s = zmk(0.0, 0.0);
+          for (int h = 0; h < n4; h += 4) {
+            zd tv[4], bv[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              tv[d] = Tc[r * PS_BS + h + d];
+              bv[d] = Bm[(h + d) * PS_BS + g];
+            }
+#pragma unroll
+            for (int d = 0; d < 4; ++d) s = zadd(s, zmul(tv[d], bv[d]));
+          }
+          C[r * PS_BS + g] = s;
+        }
+      }
+      __syncthreads();
+      const zd *__restrict__ Cm = C;
+      const zd *__restrict__ Oc = T01c;
+      zd *__restrict__ T11r = T11;
+      for (int e = tid; e < nn * nn; e += PS_THREADS) {
+        const int r = e / nn, c = e % nn;
+        zd s = T11r[r * PS_BS + c];
+        for (int g = 0; g < n4; g += 4) {
+          zd cv[4], ov[4];
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            cv[d] = Cm[r * PS_BS + g + d];
+            ov[d] = Oc[(g + d) * PS_BS + c];
+          }
+#pragma unroll
+          for (int d = 0; d < 4; ++d) s = zsub(s, zmul(cv[d], ov[d]));
+        }
+        T11r[r * PS_BS + c] = s;
+      }
     }
+    if (tid == 0) {
+      L.npub = 0;
+      L.done = 0;
+    }
+    __syncthreads();
     // the window of block k+1 is T11; the old window's buffer becomes T11
     const int tmp = iW;
     iW = i11;
@@ -441,24 +643,20 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     atomicAdd(&A.stats[0], (unsigned long long)total_acc);
     atomicAdd(&A.stats[1], (unsigned long long)total_rej);
   }
+  PROF_FLUSH();
 }
 
 // --------------------------------------------------------------------------
 // Helpers: own columns, wraps, block steps.
 // --------------------------------------------------------------------------
 struct ps_helper_hdr {
-  zd sA[PS_BS * PS_BS];        // A (row-major, 4a x 4a)
-  zd sT[PS_BS * 4];            // A-recurrence temporaries
-  zd sX[PS_NB][16];
-  zd sUB[PS_NB * (PS_NB - 1) / 2][16];  // u_{m'}[S_m] for m' < m
-  zd sVB[PS_NB * (PS_NB - 1) / 2][16];  // v_{m'}[S_m]
+  zd sB[PS_BS * PS_BS];        // B (row-major, stride PS_BS)
   int li[PS_NB];
   int a, ok;
 };
+// dynamic part of a helper's LDS after the header: sw, sy [4 bph][PS_BS], sC [4 bph][Ns]
 
-static __device__ __forceinline__ int ps_pair(int mp, int m) { return m * (m - 1) / 2 + mp; }
-
-// Own columns: row r of local column q at sC[q * Ns + r]; sv/sy are [ncol][PS_BS].
+// Own columns: row r of local column q at sC[q * Ns + r].
 static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bool forward) {
   const int Ns = A.md.Ns, npla = A.md.npla, tid = threadIdx.x;
   const ekv_t *ek = A.ekv + ((size_t)(A.t - 1) * 4 + A.f) * npla;
@@ -496,7 +694,6 @@ static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bo
 static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, int b0, int b1, int kstep) {
   // after step kstep the slot holds the columns of blocks kstep+1, kstep+2
   const int Ns = A.md.Ns, tid = threadIdx.x;
-  const int s = ps_sidx(kstep);
   for (int b = b0; b < b1; ++b) {
     const int blk = b / A.nb;
     const int mb = blk - (kstep + 1);
@@ -504,17 +701,52 @@ static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, int b0, i
     const int cl0 = 4 * (b - blk * A.nb);
     for (int e = tid; e < 4 * Ns; e += PS_THREADS) {
       const int c = e / Ns, r = e % Ns;
-      st_coh(ps_scol(A, s, mb, cl0 + c) + r, sC[(4 * (b - b0) + c) * Ns + r]);
+      st_coh(ps_scol(A, kstep, mb, cl0 + c) + r, sC[(4 * (b - b0) + c) * Ns + r]);
     }
   }
 }
 
-static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sv, zd *sy) {
-  const int tid = threadIdx.x, lane = tid & 63;
+// G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own NC columns.  All pivot
+// loads of a row are issued before the first use; entries past 4a read one
+// fixed word (a single request per wave) and are not used.
+template <int NC>
+static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4, const zd *sy,
+                                       zd *sC) {
+  const int Ns = A.md.Ns;
+  constexpr int HB = PS_BS / 2;
+  for (int r = threadIdx.x; r < Ns; r += PS_THREADS) {
+    zd acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) acc[q] = zmk(0.0, 0.0);
+    for (int g0 = 0; g0 < n4; g0 += HB) {
+      zd P[HB];
+#pragma unroll
+      for (int i = 0; i < HB; ++i) {
+        const int g = g0 + i;
+        const zd *p = g < n4 ? ps_scol(A, kstep, 0, 4 * Hh.li[g >> 2] + (g & 3)) + r : A.slot;
+        P[i] = *p;
+      }
+#pragma unroll
+      for (int i = 0; i < HB; ++i) {
+        if (g0 + i < n4) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) acc[q] = zadd(acc[q], zmul(P[i], sy[q * PS_BS + g0 + i]));
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) sC[q * Ns + r] = zadd(sC[q * Ns + r], acc[q]);
+  }
+}
+
+template <int BPH>
+static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy) {
+  constexpr int NC = 4 * BPH;
+  const int tid = threadIdx.x;
   const model_t &md = A.md;
   const int Ns = md.Ns, npla = md.npla;
   const int h = blockIdx.x - 1;
-  const int b0 = h * A.bph, b1 = min(npla, b0 + A.bph);
+  const int b0 = h * BPH, b1 = min(npla, b0 + BPH);
   const int ncol = 4 * (b1 - b0);
   PROF_T0();
   // ---- load own columns (natural column = site), optional backward wrap ----
@@ -522,6 +754,7 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     const int q = e / Ns, r = e % Ns;
     sC[e] = A.G[r + (size_t)Ns * A.fsites[4 * b0 + q]];
   }
+  for (int e = ncol * Ns + tid; e < NC * Ns; e += PS_THREADS) sC[e] = zmk(0.0, 0.0);
   __syncthreads();
   if (A.wrap == 2) ps_wrap_cols(A, sC, b0, b1, false);
   ps_publish_cols(A, sC, b0, b1, -1);
@@ -531,122 +764,64 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
   PROF_ACC(0);
   for (int k = 0; k < A.nblk; ++k) {
     if (tid == 0) {
-      Hh.ok = ps_wait(A, &A.sync[0], (unsigned long long)(k + 1), ps_arr(A, k - 1), (unsigned long long)A.H) ? 1 : 0;
-      Hh.a = Hh.ok ? ld_coh_i(&A.rec[k].a) : 0;
+      const unsigned long long fl = ps_wait(A, ps_flag(A, k), 1ull, ps_arr(A, k - 1), (unsigned long long)A.H);
+      Hh.ok = fl != 0;
+      Hh.a = fl != 0 ? (int)fl - 1 : 0;
     }
     __syncthreads();
     if (!Hh.ok) break;
     PROF_ACC(1);
-    const int a = Hh.a;
+    const int a = Hh.a, n4 = 4 * a;
     if (a > 0) {
       const ps_rec *R = A.rec + k;
-      if (tid < a) Hh.li[tid] = ld_coh_i(&R->li[tid]);
-      if (tid < 16 * a) Hh.sX[tid >> 4][tid & 15] = ld_coh(&R->X[tid >> 4][tid & 15]);
-      __syncthreads();
-      const int npair = a * (a - 1) / 2;
-      for (int e = tid; e < npair * 16; e += PS_THREADS) {
-        const int pr = e >> 4, q = e & 15;
-        int m = 1;
-        while (m * (m + 1) / 2 <= pr) ++m;
-        const int mp = pr - m * (m - 1) / 2;
-        Hh.sUB[pr][q] = ld_coh(&R->ub[mp][Hh.li[m]][q]);
-        Hh.sVB[pr][q] = ld_coh(&R->vb[mp][Hh.li[m]][q]);
-      }
-      __syncthreads();
-      const int n4 = 4 * a;
-      const int *Sk = A.fsites + 4 * A.nb * k;
-      // v for own columns: threads (q, c), sequential over m (one wave covers all)
-      if (tid < 4 * ncol) {
-        const int q = tid >> 2, c = tid & 3;
-        const int qsite = A.fsites[4 * b0 + q];
-        for (int m = 0; m < a; ++m) {
-          const int s = Sk[4 * Hh.li[m] + c];
-          zd val = zsub(zmk(qsite == s ? 1.0 : 0.0, 0.0), sC[q * Ns + s]);
-          for (int mp = 0; mp < m; ++mp) {
-            const zd *ub = Hh.sUB[ps_pair(mp, m)];
-#pragma unroll
-            for (int d = 0; d < 4; ++d) val = zsub(val, zmul(ub[c * 4 + d], sv[q * PS_BS + 4 * mp + d]));
-          }
-          sv[q * PS_BS + 4 * m + c] = val;
-          wave_sync();
-        }
-      }
-      // A: threads (e, kk), e < 4a; block column m = (sel_m + sum A[:, m'] v_{m'}[S_m]) X_m
-      for (int m = 0; m < a; ++m) {
-        if (tid < n4 * 4) {
-          const int e = tid >> 2, kk = tid & 3;
-          zd t = zmk(e == 4 * m + kk ? 1.0 : 0.0, 0.0);
-          for (int mp = 0; mp < m; ++mp) {
-            const zd *vb = Hh.sVB[ps_pair(mp, m)];
-#pragma unroll
-            for (int d = 0; d < 4; ++d) t = zadd(t, zmul(Hh.sA[e * PS_BS + 4 * mp + d], vb[d * 4 + kk]));
-          }
-          Hh.sT[e * 4 + kk] = t;
-        }
-        __syncthreads();
-        if (tid < n4 * 4) {
-          const int e = tid >> 2, c = tid & 3;
-          zd s = zmk(0.0, 0.0);
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) s = zadd(s, zmul(Hh.sT[e * 4 + kk], Hh.sX[m][kk + 4 * c]));
-          Hh.sA[e * PS_BS + 4 * m + c] = s;
-        }
-        __syncthreads();
-      }
-      // y[q] = A v[q]
-      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
-        const int q = e / n4, r = e % n4;
-        zd s = zmk(0.0, 0.0);
-        for (int f = r & ~3; f < n4; ++f) s = zadd(s, zmul(Hh.sA[r * PS_BS + f], sv[q * PS_BS + f]));
-        sy[q * PS_BS + r] = s;
-      }
+      if (tid < a) Hh.li[tid] = R->li[tid];
+      for (int e = tid; e < n4 * n4; e += PS_THREADS) Hh.sB[(e / n4) * PS_BS + e % n4] = R->B[e];
       __syncthreads();
       PROF_ACC(2);
-      // G[:, q] += G_k[:, S_A] y[q]; pivot columns from the slot of step k-1
-      const int s = ps_sidx(k - 1);
-      for (int r = tid; r < Ns; r += PS_THREADS) {
-        zd accq[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) accq[q] = zmk(0.0, 0.0);
-        for (int f0 = 0; f0 < n4; f0 += 16) {
-          zd P[16];
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int f = f0 + i;
-            P[i] = (f < n4) ? ld_coh(ps_scol(A, s, 0, 4 * Hh.li[f >> 2] + (f & 3)) + r) : zmk(0.0, 0.0);
-          }
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int f = f0 + i;
-            if (f < n4) {
-#pragma unroll
-              for (int q = 0; q < 8; ++q)
-                if (q < ncol) accq[q] = zadd(accq[q], zmul(P[i], sy[q * PS_BS + f]));
-            }
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (q < ncol) sC[q * Ns + r] = zadd(sC[q * Ns + r], accq[q]);
+      // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q]
+      const int *Sk = A.fsites + 4 * A.nb * k;
+      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
+        const int q = e / n4, g = e % n4;
+        const int st = Sk[4 * Hh.li[g >> 2] + (g & 3)];
+        sw[q * PS_BS + g] = zsub(zmk(A.fsites[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
       }
       __syncthreads();
+      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
+        const int q = e / n4, g = e % n4;
+        zd y = zmk(0.0, 0.0);
+        for (int hh = 0; hh < n4; hh += 4) {
+          zd bv[4], wv[4];
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            bv[d] = Hh.sB[g * PS_BS + hh + d];
+            wv[d] = sw[q * PS_BS + hh + d];
+          }
+#pragma unroll
+          for (int d = 0; d < 4; ++d) y = zadd(y, zmul(bv[d], wv[d]));
+        }
+        sy[q * PS_BS + g] = y;
+      }
+      __syncthreads();
+      PROF_ACC(3);
+      // G[:, q] += G_k[:, S_A] y[q]; pivot columns from the slot of step k-1
+      ps_pivot_update<NC>(A, Hh, k - 1, n4, sy, sC);
+      __syncthreads();
     }
-    PROF_ACC(3);
-    ps_publish_cols(A, sC, b0, b1, k);
     PROF_ACC(5);
+    ps_publish_cols(A, sC, b0, b1, k);
     ps_release();
-    PROF_ACC(6);
     __syncthreads();
+    PROF_ACC(6);
     if (tid == 0) atomicAdd(ps_arr(A, k), 1ull);
     PROF_ACC(7);
   }
-  (void)lane;
   if (A.wrap == 1) ps_wrap_cols(A, sC, b0, b1, true);
   for (int e = tid; e < ncol * Ns; e += PS_THREADS) {
     const int q = e / Ns, r = e % Ns;
     A.G[r + (size_t)Ns * A.fsites[4 * b0 + q]] = sC[e];
   }
   PROF_ACC(4);
+  PROF_FLUSH();
 }
 
 __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
@@ -661,12 +836,13 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     ps_leader(A, *reinterpret_cast<ps_leader_lds *>(ps_lds));
   } else {
     ps_helper_hdr &Hh = *reinterpret_cast<ps_helper_hdr *>(ps_lds);
-    zd *base = reinterpret_cast<zd *>(ps_lds + sizeof(ps_helper_hdr));
-    const int ncol_max = 4 * A.bph;
-    zd *sv = base;
-    zd *sy = sv + ncol_max * PS_BS;
-    zd *sC = sy + ncol_max * PS_BS;
-    ps_helper(A, Hh, sC, sv, sy);
+    zd *sw = reinterpret_cast<zd *>(ps_lds + sizeof(ps_helper_hdr));
+    zd *sy = sw + 4 * A.bph * PS_BS;
+    zd *sC = sy + 4 * A.bph * PS_BS;
+    if (A.bph == 1)
+      ps_helper<1>(A, Hh, sC, sw, sy);
+    else
+      ps_helper<2>(A, Hh, sC, sw, sy);
   }
   // the last workgroup to finish clears the sync words for the next launch
   __syncthreads();
@@ -674,9 +850,9 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     ps_release();
     const unsigned long long done = atomicAdd(&A.sync[1], 1ull);
     if (done == gridDim.x - 1) {
-      ps_st(&A.sync[0], 0ull);
       ps_st(&A.sync[2], 0ull);
       for (int k = -1; k < A.nblk; ++k) ps_st(ps_arr(A, k), 0ull);
+      for (int k = 0; k < A.nblk; ++k) ps_st(ps_flag(A, k), 0ull);
       ps_st(&A.sync[1], 0ull);
     }
   }
@@ -690,6 +866,7 @@ extern "C" size_t pqmc_pass_lds(int Ns, int bph) {
 extern "C" size_t pqmc_pass_rec_bytes(void) { return sizeof(ps_rec); }
 
 extern "C" hipError_t pqmc_launch_pass(const ps_args *a, hipStream_t s) {
+  if (a->bph < 1 || a->bph > 2) return hipErrorInvalidValue;
   const size_t lds = pqmc_pass_lds(a->md.Ns, a->bph);
   static size_t attr = 0;
   if (lds > attr) {

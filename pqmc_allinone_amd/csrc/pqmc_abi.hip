@@ -248,9 +248,9 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->qm.beta, ne);
   A_(h->qm.vn2, ne);
   A_(h->d_acc, (size_t)h->Nb * h->Lt);
-  A_(h->d_slot, (size_t)2 * 2 * PS_BS * Ns);
+  A_(h->d_slot, pqmc_pass_slot_elems(Ns, h->nblk));
   A_(h->d_rec, (size_t)h->nblk);
-  A_(h->d_psync, (size_t)h->nblk + 16);
+  A_(h->d_psync, pqmc_pass_sync_words(h->nblk));
   A_(h->d_lncw_acc, 1);
   A_(h->d_lncw, 1);
   A_(h->d_stats, 4);
@@ -259,7 +259,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
 #undef A_
   if (hipMemsetAsync(h->d_dlog, 0, h->nblock * sizeof(zd), h->st) != hipSuccess ||
-      hipMemsetAsync(h->d_psync, 0, (h->nblk + 16) * sizeof(unsigned long long), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_psync, 0, pqmc_pass_sync_words(h->nblk) * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
@@ -541,6 +541,7 @@ static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u,
   a.u = d_u;
   a.G = h->d_G;
   a.slot = h->d_slot;
+  a.lds = pqmc_pass_slot_ld(h->Ns);
   a.rec = h->d_rec;
   a.sync = h->d_psync;
   a.acc = d_acc;
