@@ -128,11 +128,10 @@ void pqmc_ran2_seed(pqmc_ran2 *s, int32_t seed);
 void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out);
 
 /* Debug phase timers of the family-pass kernel: enable allocates and zeroes
- * 8 accumulators per workgroup; out (8*P doubles, microseconds) and P (the
- * workgroup count) may be NULL.  Workgroup 0 (decisions): set-up, decisions,
- * wait for the next block's tiles, replay.  Others (column owners): load and
- * wrap, wait, block-step set-up, update and publish, final wrap and store.
- * No reference counterpart (diagnostics). */
+ * 8 accumulators per row; out (8*P doubles, microseconds) and P (the row
+ * count) may be NULL.  Rows: the deciding wave of workgroup 0, then one per
+ * column-owning workgroup, then workgroup 0's B-building and tile-loading
+ * waves.  No reference counterpart (diagnostics). */
 int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P);
 
 /* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */

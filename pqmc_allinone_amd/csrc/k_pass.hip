@@ -144,6 +144,18 @@ static __device__ __forceinline__ int ps_nsites(const ps_args &A, int k) {
   return 4 * min(A.nb, A.md.npla - k * A.nb);
 }
 
+// acc + a * b with four FMAs (throughput form for long accumulations)
+static __device__ __forceinline__ zd zfma(zd acc, zd a, zd b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+  return acc;
+}
+
+// v where ok, else zero (a select on the components, not on addresses)
+static __device__ __forceinline__ zd zsel(bool ok, zd v) { return zmk(ok ? v.x : 0.0, ok ? v.y : 0.0); }
+
 static __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -155,11 +167,6 @@ static __device__ __forceinline__ double row4_sum(double v) {
   v += dpp_move<0x124, 0xF>(v);  // row_ror:4
   v += dpp_move<0x128, 0xF>(v);  // row_ror:8
   return v;
-}
-static __device__ __forceinline__ double lane0(double v) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 0);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 0);
-  return __hiloint2double(hi, lo);
 }
 
 // modi of src/mod_matrixlib.f90:66-77
@@ -195,10 +202,20 @@ static __device__ __forceinline__ zd minor3(const zd *M, int ri, int cj) {
       _pt = _now;                                                \
     }                                                            \
   } while (0)
+#define PROF_FLUSH_ROW(row)                                                               \
+  do {                                                                                    \
+    if (A.prof && (threadIdx.x & 63) == 0)                                                \
+      for (int _k = 0; _k < 8; ++_k) A.prof[(row) * 8 + _k] += _pa[_k];                   \
+  } while (0)
+// rows: 0 leader wave 0, 1..H helpers, H+1 leader wave 1, H+2 leader wave 2
 #define PROF_FLUSH()                                                                      \
   do {                                                                                    \
-    if (A.prof && threadIdx.x == 0)                                                       \
-      for (int _k = 0; _k < 8; ++_k) A.prof[blockIdx.x * 8 + _k] += _pa[_k];              \
+    if (threadIdx.x == 0)                                                                 \
+      PROF_FLUSH_ROW(blockIdx.x);                                                         \
+    else if (blockIdx.x == 0 && threadIdx.x == 64)                                        \
+      PROF_FLUSH_ROW(A.H + 1);                                                            \
+    else if (blockIdx.x == 0 && threadIdx.x == 128)                                       \
+      PROF_FLUSH_ROW(A.H + 2);                                                            \
   } while (0)
 
 // Proposal of bond p (family order): jump, bosonic action change
@@ -254,6 +271,39 @@ static __device__ void ps_load_tile(const ps_args &A, int kstep, int rb, int cb,
   }
 }
 
+// The three tiles that fold block k into the window of block k+1, from the
+// slot written after step k-1: T10 = G[S_{k+1}, S_k] stored transposed
+// (T10t[c * PS_BS + r]), T01 = G[S_k, S_{k+1}] and T11 = G[S_{k+1}, S_{k+1}]
+// row-major; entries outside the blocks are zero.  256 threads (t0 = 0..255)
+// issue all their loads before the first LDS store.
+static __device__ void ps_load_fold_tiles(const ps_args &A, int k, zd *T10t, zd *T01, zd *T11, int t0) {
+  // thread t0 takes column c = t0 % 32 and rows r0 + 8j (j = 0..3) of each tile
+  const int n0 = ps_nsites(A, k), n1 = ps_nsites(A, k + 1);
+  const int c = t0 & 31, r0 = t0 >> 5;
+  const int *S0 = A.fsites + 4 * A.nb * k, *S1 = S0 + 4 * A.nb;
+  const zd *col0 = ps_scol(A, k - 1, 0, c), *col1 = ps_scol(A, k - 1, 1, c);
+  int s0[4], s1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    s0[j] = S0[min(r0 + 8 * j, n0 - 1)];
+    s1[j] = S1[min(r0 + 8 * j, n1 - 1)];
+  }
+  zd v10[4], v01[4], v11[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v10[j] = col0[s1[j]];  // G[S_{k+1}(r), S_k(c)]
+    v01[j] = col1[s0[j]];  // G[S_k(r), S_{k+1}(c)]
+    v11[j] = col1[s1[j]];  // G[S_{k+1}(r), S_{k+1}(c)]
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = r0 + 8 * j;
+    T10t[c * PS_BS + r] = zsel(r < n1 && c < n0, v10[j]);
+    T01[r * PS_BS + c] = zsel(r < n0 && c < n1, v01[j]);
+    T11[r * PS_BS + c] = zsel(r < n1 && c < n1, v11[j]);
+  }
+}
+
 // --------------------------------------------------------------------------
 // Leader: decisions (wave 0), B (wave 1), next block (waves 2-7), fold.
 // --------------------------------------------------------------------------
@@ -266,12 +316,31 @@ struct ps_leader_lds {
   zd sTi[PS_BS * PS_BS];        // T^-1 (row-major, stride PS_BS)
   zd sB[PS_BS * PS_BS];         // B = A T^-1 (row-major, stride PS_BS)
   zd sTt[PS_BS * 4];
-  zd sM[16], sCof[16];
+  zd sM[4][16], sCof[16];
   zd dets[PS_NB];
   ps_prop prop[2][PS_NB];
   int li[PS_NB];
-  int npub, done, nacc, abort;
+  int npub, done, nacc, abort, fold_ctr;
 };
+
+// Barrier among the n waves that call it, through an LDS counter that the
+// caller resets between uses (lane 0 of each wave counts and waits).
+static __device__ void ps_lds_bar(const ps_args &A, int *ctr, int n) {
+  if ((threadIdx.x & 63) == 0) {
+    __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const unsigned long long t0 = wall_clock64();
+    for (unsigned it = 1; __hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < n; ++it) {
+      if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
+        __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(&A.stats[2], 1ull);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  wave_sync();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 // Wave 1: fold accept m of the current block into A, T^-1 and B.
 static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
@@ -362,6 +431,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     L.abort = 0;
     L.npub = 0;
     L.done = 0;
+    L.fold_ctr = 0;
     L.nacc = ps_wait(A, ps_arr(A, -1), (unsigned long long)A.H, nullptr, 0) ? 1 : 0;
   }
   __syncthreads();
@@ -380,34 +450,53 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       zd *W = L.buf[iW];
       const ps_prop *pp = L.prop[k & 1];
       int m = 0;
-      for (int j = 0; j < nbk; ++j) {
-        // M = I + G_SS Delta (src/mod_update.f90:97-100), lane a + 4c holds M(a, c)
-        const int ra = lane & 3, rc = (lane >> 2) & 3;
-        if (lane < 16) {
+      for (int jb = 0; jb < nbk;) {
+        // One round evaluates the next four bonds from the same window, one
+        // per 16-lane row (bond jb + gq).  Bonds before the first accepted one
+        // are rejected exactly as in the sequential scan, since a rejection
+        // leaves G unchanged; the round then continues after that bond.
+        // M = I + G_SS Delta (src/mod_update.f90:97-100), lane a + 4c of a row holds M(a, c)
+        const int gq = lane >> 4, ll = lane & 15, ra = ll & 3, rc = ll >> 2;
+        const int nv = min(4, nbk - jb);
+        const int jq = jb + min(gq, nv - 1);
+        {
+          zd wv[4], dv[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            wv[kk] = W[(4 * jq + ra) * PS_BS + 4 * jq + kk];
+            dv[kk] = pp[jq].D[kk + 4 * rc];
+          }
           zd mv = zmk(ra == rc ? 1.0 : 0.0, 0.0);
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk) mv = zadd(mv, zmul(W[(4 * j + ra) * PS_BS + 4 * j + kk], pp[j].D[kk + 4 * rc]));
-          L.sM[lane] = mv;
+          for (int kk = 0; kk < 4; ++kk) mv = zadd(mv, zmul(wv[kk], dv[kk]));
+          L.sM[gq][ll] = mv;
         }
         wave_sync();
-        // cofactor C(ra, rc); det = sum_c M(0, c) C(0, c) over lanes 0, 4, 8, 12
-        zd cof = zmk(0.0, 0.0), term = zmk(0.0, 0.0);
-        if (lane < 16) {
-          cof = minor3(L.sM, ra, rc);
-          if ((ra + rc) & 1) cof = zneg(cof);
-          if (ra == 0) term = zmul(L.sM[lane], cof);
-        }
-        const double dre = lane0(row4_sum(term.x)), dim = lane0(row4_sum(term.y));
-        const double n2 = dre * dre + dim * dim;
-        const bool acc = !(n2 < pp[j].thr);
-        if (lane == 0) A.acc[p0 + j] = (uint8_t)acc;
+        // cofactor C(ra, rc); det = sum_c M(0, c) C(0, c) over lanes 0, 4, 8, 12 of the row
+        zd cof = minor3(L.sM[gq], ra, rc);
+        if ((ra + rc) & 1) cof = zneg(cof);
+        const zd term = ra == 0 ? zmul(L.sM[gq][ll], cof) : zmk(0.0, 0.0);
+        const double tr = row4_sum(term.x), ti = row4_sum(term.y);
+        const bool accq = gq < nv && !(tr * tr + ti * ti < pp[jq].thr);
+        const unsigned long long bal = __ballot(ll == 0 && accq);
+        const int f = bal ? (int)(__builtin_ctzll(bal) >> 4) : nv;
+        if (ll == 0 && gq < f) A.acc[p0 + jb + gq] = 0;
+        total_rej += f;
         PROF_ACC(4);
-        if (!acc) {
-          ++total_rej;
+        if (f == nv) {
+          jb += nv;
           continue;
         }
+        const int j = jb + f;
+        jb = j + 1;
+        const double dre = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(tr), 16 * f),
+                                            __builtin_amdgcn_readlane(__double2loint(tr), 16 * f));
+        const double dim = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(ti), 16 * f),
+                                            __builtin_amdgcn_readlane(__double2loint(ti), 16 * f));
+        const double n2 = dre * dre + dim * dim;
+        if (lane == 0) A.acc[p0 + j] = 1;
+        if (gq == f) L.sCof[ll] = cof;
         // X = Delta M^-1 with M^-1(kk, c) = C(c, kk) / det
-        if (lane < 16) L.sCof[lane] = cof;
         wave_sync();
         if (lane < 16) {
           zd dv[4], cv[4];
@@ -480,11 +569,13 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
             for (int i2 = 0; i2 < 4; ++i2) w[i2] = Wr[r * PS_BS + min(o + cl + 8 * i2, PS_BS - 1)];
 #pragma unroll
             for (int i2 = 0; i2 < 4; ++i2) {
-              const int c = o + cl + 8 * i2;
-              zd s = w[i2];
+              if (i2 < ni) {
+                const int c = o + cl + 8 * i2;
+                zd s = w[i2];
 #pragma unroll
-              for (int a = 0; a < 4; ++a) s = zadd(s, zmul(uu[a], vv[i2][a]));
-              if (rv && i2 < ni && c < nsk) Wr[r * PS_BS + c] = s;
+                for (int a = 0; a < 4; ++a) s = zadd(s, zmul(uu[a], vv[i2][a]));
+                if (rv && c < nsk) Wr[r * PS_BS + c] = s;
+              }
             }
           }
         }
@@ -498,11 +589,12 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         __hip_atomic_store(&L.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       ps_release();  // acc flags, fields and exponentials are complete before block k is published
-      PROF_ACC(1);
+      PROF_ACC(0);
     } else if (wave == 1) {
       // ---- B of block k, accept by accept ----
       for (int e = lane; e < PS_BS * PS_BS; e += 64) L.sB[e] = zmk(0.0, 0.0);
       wave_sync();
+      PROF_ACC(3);
       int md_ = 0;
       const unsigned long long t0 = wall_clock64();
       for (unsigned it = 1;; ++it) {
@@ -526,23 +618,26 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
             continue;
           }
         }
+        PROF_ACC(0);
         for (; md_ < np; ++md_) ps_fold_accept(L, md_, lane);
+        PROF_ACC(1);
       }
-    } else if (has_next) {
-      // ---- waves 2..7: next block's proposals and tiles after step k-1 ----
-      const int t1 = tid - 128, nt1 = PS_THREADS - 128;
-      const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
-      if (t1 < nb1) ps_propose(A, p1 + t1, L.prop[(k + 1) & 1][t1]);
+      PROF_ACC(0);
+    } else if (has_next && wave <= 5) {
+      // ---- waves 2..5: the fold tiles after step k-1 ----
       bool wok = true;
       if (lane == 0) wok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
       wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
-      if (wok) {
-        ps_load_tile(A, k - 1, k + 1, k, L.buf[i10], t1, nt1);
-        ps_load_tile(A, k - 1, k, k + 1, L.buf[i01], t1, nt1);
-        ps_load_tile(A, k - 1, k + 1, k + 1, L.buf[i11], t1, nt1);
-      } else if (lane == 0) {
+      PROF_ACC(1);
+      if (wok)
+        ps_load_fold_tiles(A, k, L.buf[i10], L.buf[i01], L.buf[i11], tid - 128);
+      else if (lane == 0)
         L.abort = 1;
-      }
+      PROF_ACC(7);
+    } else if (has_next && wave == 7) {
+      // ---- wave 7: the next block's proposals ----
+      const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
+      if (lane < nb1) ps_propose(A, p1 + lane, L.prop[(k + 1) & 1][lane]);
     }
     if (tid == 0 && ps_ld(&A.sync[2]) != 0) L.abort = 1;
     __syncthreads();
@@ -567,72 +662,79 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       }
       lv = wave_sum(lv);
       if (lane == 0 && a > 0) atomicAdd(A.lncw_acc, lv);
+      PROF_ACC(4);
     }
     if (!has_next) break;
     const int nn = ps_nsites(A, k + 1);
     zd *T10 = L.buf[i10], *T01 = L.buf[i01], *T11 = L.buf[i11];
-    if (a > 0) {
+    if (a > 0 && (wave == 0 || (wave >= 2 && wave <= 4))) {
       // ---- fold block k into the next window: T11 -= (T10[:, S_A] B) T01[S_A, :] ----
-      // gather the accepted columns of T10 and rows of T01 (wave 1 is publishing)
-      zd *T10c = L.sTi, *T01c = L.buf[iW], *C = L.sA;
-      if (wave != 1) {
-        const int t2 = wave == 0 ? tid : tid - 64;
-        for (int e = t2; e < 2 * nn * n4; e += PS_THREADS - 64) {
-          const int e2 = e < nn * n4 ? e : e - nn * n4;
-          const int x = e2 / n4, g = e2 % n4;
-          const int cg = 4 * L.li[g >> 2] + (g & 3);
-          if (e < nn * n4)
-            T10c[x * PS_BS + g] = T10[x * PS_BS + cg];
-          else
-            T01c[g * PS_BS + x] = T01[cg * PS_BS + x];
+      // on FP64 MFMA, one 16 x 16 tile per wave (waves 0, 2, 3, 4) while wave 1
+      // publishes; lane l supplies A[l%16][l/16] and B[l/16][l%16], and holds
+      // rows l/16 + 4i of column l%16 of the result.
+      const int tt = wave == 0 ? 0 : wave - 1;
+      const int rt = tt >> 1, ct = tt & 1;
+      const int lr = lane & 15, lk = lane >> 4;
+      zd *C = L.sA;
+      if (16 * ct < n4) {
+        // two k-steps per round into two interleaved MFMA chains
+        double4_t cre0 = {0.0, 0.0, 0.0, 0.0}, cim0 = cre0, cre1 = cre0, cim1 = cre0;
+        for (int ks = 0; ks < a; ks += 2) {
+          const int k2 = min(ks + 1, a - 1);
+          const zd a0 = T10[(4 * L.li[ks] + lk) * PS_BS + 16 * rt + lr];  // T10 is stored transposed
+          const zd b0 = L.sB[(4 * ks + lk) * PS_BS + 16 * ct + lr];
+          const zd a1 = T10[(4 * L.li[k2] + lk) * PS_BS + 16 * rt + lr];
+          const zd b1 = zsel(ks + 1 < a, L.sB[(4 * k2 + lk) * PS_BS + 16 * ct + lr]);
+          cre0 = mfma_f64(a0.x, b0.x, cre0);
+          cim0 = mfma_f64(a0.x, b0.y, cim0);
+          cre1 = mfma_f64(a1.x, b1.x, cre1);
+          cim1 = mfma_f64(a1.x, b1.y, cim1);
+          cre0 = mfma_f64(-a0.y, b0.y, cre0);
+          cim0 = mfma_f64(a0.y, b0.x, cim0);
+          cre1 = mfma_f64(-a1.y, b1.y, cre1);
+          cim1 = mfma_f64(a1.y, b1.x, cim1);
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          C[(16 * ct + lr) * PS_BS + 16 * rt + lk + 4 * i] = zmk(cre0[i] + cre1[i], cim0[i] + cim1[i]);  // transposed
       }
-      __syncthreads();
-      if (wave != 1) {
-        const int t2 = wave == 0 ? tid : tid - 64;
-        const zd *__restrict__ Tc = T10c;
-        const zd *__restrict__ Bm = L.sB;
-        for (int e = t2; e < nn * n4; e += PS_THREADS - 64) {
-          const int r = e / n4, g = e % n4;
-          zd s = zmk(0.0, 0.0);
-          for (int h = 0; h < n4; h += 4) {
-            zd tv[4], bv[4];
+      ps_lds_bar(A, &L.fold_ctr, 4);
+      double4_t dre0, dim0, dre1 = {0.0, 0.0, 0.0, 0.0}, dim1 = dre1;
 #pragma unroll
-            for (int d = 0; d < 4; ++d) {
-              tv[d] = Tc[r * PS_BS + h + d];
-              bv[d] = Bm[(h + d) * PS_BS + g];
-            }
-#pragma unroll
-            for (int d = 0; d < 4; ++d) s = zadd(s, zmul(tv[d], bv[d]));
-          }
-          C[r * PS_BS + g] = s;
-        }
+      for (int i = 0; i < 4; ++i) {
+        const zd t = T11[(16 * rt + lk + 4 * i) * PS_BS + 16 * ct + lr];
+        dre0[i] = t.x;
+        dim0[i] = t.y;
       }
-      __syncthreads();
-      const zd *__restrict__ Cm = C;
-      const zd *__restrict__ Oc = T01c;
-      zd *__restrict__ T11r = T11;
-      for (int e = tid; e < nn * nn; e += PS_THREADS) {
-        const int r = e / nn, c = e % nn;
-        zd s = T11r[r * PS_BS + c];
-        for (int g = 0; g < n4; g += 4) {
-          zd cv[4], ov[4];
-#pragma unroll
-          for (int d = 0; d < 4; ++d) {
-            cv[d] = Cm[r * PS_BS + g + d];
-            ov[d] = Oc[(g + d) * PS_BS + c];
-          }
-#pragma unroll
-          for (int d = 0; d < 4; ++d) s = zsub(s, zmul(cv[d], ov[d]));
-        }
-        T11r[r * PS_BS + c] = s;
+      for (int ks = 0; ks < a; ks += 2) {
+        const int k2 = min(ks + 1, a - 1);
+        const zd a0 = C[(4 * ks + lk) * PS_BS + 16 * rt + lr];
+        const zd b0 = T01[(4 * L.li[ks] + lk) * PS_BS + 16 * ct + lr];
+        const zd a1 = zsel(ks + 1 < a, C[(4 * k2 + lk) * PS_BS + 16 * rt + lr]);
+        const zd b1 = T01[(4 * L.li[k2] + lk) * PS_BS + 16 * ct + lr];
+        dre0 = mfma_f64(-a0.x, b0.x, dre0);
+        dim0 = mfma_f64(-a0.x, b0.y, dim0);
+        dre1 = mfma_f64(-a1.x, b1.x, dre1);
+        dim1 = mfma_f64(-a1.x, b1.y, dim1);
+        dre0 = mfma_f64(a0.y, b0.y, dre0);
+        dim0 = mfma_f64(-a0.y, b0.x, dim0);
+        dre1 = mfma_f64(a1.y, b1.y, dre1);
+        dim1 = mfma_f64(-a1.y, b1.x, dim1);
       }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        T11[(16 * rt + lk + 4 * i) * PS_BS + 16 * ct + lr] = zmk(dre0[i] + dre1[i], dim0[i] + dim1[i]);
     }
+    if (wave == 0)
+      PROF_ACC(1);
+    else
+      PROF_ACC(6);
     if (tid == 0) {
       L.npub = 0;
       L.done = 0;
     }
     __syncthreads();
+    if (tid == 0) L.fold_ctr = 0;  // all fold waves have left their LDS barrier
     // the window of block k+1 is T11; the old window's buffer becomes T11
     const int tmp = iW;
     iW = i11;
@@ -652,9 +754,10 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
 struct ps_helper_hdr {
   zd sB[PS_BS * PS_BS];        // B (row-major, stride PS_BS)
   int li[PS_NB];
-  int a, ok;
+  int a, ok, sink;
 };
-// dynamic part of a helper's LDS after the header: sw, sy [4 bph][PS_BS], sC [4 bph][Ns]
+// after the header: sw, sy [4 bph][PS_BS], sC [4 bph][Ns], then the family's
+// sites [4 npla] (int)
 
 // Own columns: row r of local column q at sC[q * Ns + r].
 static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bool forward) {
@@ -706,41 +809,56 @@ static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, int b0, i
   }
 }
 
-// G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own NC columns.  All pivot
-// loads of a row are issued before the first use; entries past 4a read one
-// fixed word (a single request per wave) and are not used.
-template <int NC>
-static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4, const zd *sy,
-                                       zd *sC) {
+// G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own columns, four at a
+// time.  All pivot loads of a row are issued before the first use (entries
+// past 4a read one fixed word, a single request per wave, and are not used);
+// y is staged in registers four pivots at a time.
+static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4, int ncol,
+                                       const zd *sy, zd *sC) {
   const int Ns = A.md.Ns;
   constexpr int HB = PS_BS / 2;
+  int col[PS_NB];  // first slot column of each accepted bond
+#pragma unroll
+  for (int m = 0; m < PS_NB; ++m) col[m] = 4 * Hh.li[m];
   for (int r = threadIdx.x; r < Ns; r += PS_THREADS) {
-    zd acc[NC];
+    for (int q0 = 0; q0 < ncol; q0 += 4) {
+      zd acc[4];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) acc[q] = zmk(0.0, 0.0);
-    for (int g0 = 0; g0 < n4; g0 += HB) {
-      zd P[HB];
+      for (int q = 0; q < 4; ++q) acc[q] = zmk(0.0, 0.0);
 #pragma unroll
-      for (int i = 0; i < HB; ++i) {
-        const int g = g0 + i;
-        const zd *p = g < n4 ? ps_scol(A, kstep, 0, 4 * Hh.li[g >> 2] + (g & 3)) + r : A.slot;
-        P[i] = *p;
-      }
+      for (int hb = 0; hb < 2; ++hb) {
+        const int g0 = hb * HB;
+        if (g0 >= n4) break;
+        zd P[HB];
 #pragma unroll
-      for (int i = 0; i < HB; ++i) {
-        if (g0 + i < n4) {
+        for (int i = 0; i < HB; ++i) {
+          const int g = g0 + i;
+          const zd *p = g < n4 ? ps_scol(A, kstep, 0, col[g >> 2] + (g & 3)) + r : A.slot;
+          P[i] = *p;
+        }
 #pragma unroll
-          for (int q = 0; q < NC; ++q) acc[q] = zadd(acc[q], zmul(P[i], sy[q * PS_BS + g0 + i]));
+        for (int gg = 0; gg < HB; gg += 4) {
+          if (g0 + gg < n4) {
+            zd Y[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) Y[i][q] = sy[(q0 + q) * PS_BS + g0 + gg + i];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[q] = zfma(acc[q], P[gg + i], Y[i][q]);
+          }
         }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < NC; ++q) sC[q * Ns + r] = zadd(sC[q * Ns + r], acc[q]);
+      for (int q = 0; q < 4; ++q) sC[(q0 + q) * Ns + r] = zadd(sC[(q0 + q) * Ns + r], acc[q]);
+    }
   }
 }
 
 template <int BPH>
-static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy) {
+static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy, int *sS) {
   constexpr int NC = 4 * BPH;
   const int tid = threadIdx.x;
   const model_t &md = A.md;
@@ -755,6 +873,7 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     sC[e] = A.G[r + (size_t)Ns * A.fsites[4 * b0 + q]];
   }
   for (int e = ncol * Ns + tid; e < NC * Ns; e += PS_THREADS) sC[e] = zmk(0.0, 0.0);
+  for (int e = tid; e < 4 * npla; e += PS_THREADS) sS[e] = A.fsites[e];
   __syncthreads();
   if (A.wrap == 2) ps_wrap_cols(A, sC, b0, b1, false);
   ps_publish_cols(A, sC, b0, b1, -1);
@@ -763,8 +882,26 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
   if (tid == 0) atomicAdd(ps_arr(A, -1), 1ull);
   PROF_ACC(0);
   for (int k = 0; k < A.nblk; ++k) {
+    // Block k's columns are in the slot once step k-1 is complete; the
+    // leader's decision of block k comes later.  Meanwhile the helpers of an
+    // XCD (workgroups are spread over the 8 XCDs round-robin) touch those
+    // columns in slices, so that the pivot loads of step k hit in L2.
+    if (tid == 0) Hh.ok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
+    __syncthreads();
+    if (!Hh.ok) break;
+    {
+      // one load per 128-byte line (8 entries of a column): line ln goes to
+      // the helper whose slice is ln % 12
+      const int nsk = ps_nsites(A, k), slice = (blockIdx.x >> 3) % 12, nl = (Ns + 7) >> 3;
+      double t = 0.0;
+      for (int e = tid; e < nsk * nl; e += PS_THREADS) {
+        const int c = e / nl, ln = e % nl;
+        if (ln % 12 == slice) t += ps_scol(A, k - 1, 0, c)[8 * ln].x;
+      }
+      if (t == 1.2345e300) Hh.sink = 1;  // keeps the loads; never true for Green's function entries
+    }
     if (tid == 0) {
-      const unsigned long long fl = ps_wait(A, ps_flag(A, k), 1ull, ps_arr(A, k - 1), (unsigned long long)A.H);
+      const unsigned long long fl = ps_wait(A, ps_flag(A, k), 1ull, nullptr, 0);
       Hh.ok = fl != 0;
       Hh.a = fl != 0 ? (int)fl - 1 : 0;
     }
@@ -779,11 +916,11 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
       __syncthreads();
       PROF_ACC(2);
       // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q]
-      const int *Sk = A.fsites + 4 * A.nb * k;
+      const int *Sk = sS + 4 * A.nb * k;
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
         const int q = e / n4, g = e % n4;
         const int st = Sk[4 * Hh.li[g >> 2] + (g & 3)];
-        sw[q * PS_BS + g] = zsub(zmk(A.fsites[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
+        sw[q * PS_BS + g] = zsub(zmk(sS[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
       }
       __syncthreads();
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
@@ -797,14 +934,14 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
             wv[d] = sw[q * PS_BS + hh + d];
           }
 #pragma unroll
-          for (int d = 0; d < 4; ++d) y = zadd(y, zmul(bv[d], wv[d]));
+          for (int d = 0; d < 4; ++d) y = zfma(y, bv[d], wv[d]);
         }
         sy[q * PS_BS + g] = y;
       }
       __syncthreads();
       PROF_ACC(3);
       // G[:, q] += G_k[:, S_A] y[q]; pivot columns from the slot of step k-1
-      ps_pivot_update<NC>(A, Hh, k - 1, n4, sy, sC);
+      ps_pivot_update(A, Hh, k - 1, n4, NC, sy, sC);
       __syncthreads();
     }
     PROF_ACC(5);
@@ -839,10 +976,11 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     zd *sw = reinterpret_cast<zd *>(ps_lds + sizeof(ps_helper_hdr));
     zd *sy = sw + 4 * A.bph * PS_BS;
     zd *sC = sy + 4 * A.bph * PS_BS;
+    int *sS = reinterpret_cast<int *>(sC + 4 * A.bph * A.md.Ns);
     if (A.bph == 1)
-      ps_helper<1>(A, Hh, sC, sw, sy);
+      ps_helper<1>(A, Hh, sC, sw, sy, sS);
     else
-      ps_helper<2>(A, Hh, sC, sw, sy);
+      ps_helper<2>(A, Hh, sC, sw, sy, sS);
   }
   // the last workgroup to finish clears the sync words for the next launch
   __syncthreads();
@@ -860,7 +998,8 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
 
 extern "C" size_t pqmc_pass_lds(int Ns, int bph) {
   const size_t lead = sizeof(ps_leader_lds);
-  const size_t help = sizeof(ps_helper_hdr) + sizeof(zd) * (size_t)4 * bph * (2 * PS_BS + Ns);
+  const size_t help =
+      sizeof(ps_helper_hdr) + sizeof(zd) * (size_t)4 * bph * (2 * PS_BS + Ns) + sizeof(int) * (size_t)2 * Ns;
   return lead > help ? lead : help;
 }
 extern "C" size_t pqmc_pass_rec_bytes(void) { return sizeof(ps_rec); }

@@ -812,10 +812,11 @@ extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
 }
 
 // Phase timers of the family pass (debug): enable != 0 allocates and
-// zeroes them; out (8*P values, may be NULL) receives microseconds.
+// zeroes them; out (8*P values, may be NULL) receives microseconds.  Rows:
+// leader wave 0, the H helpers, leader wave 1, leader wave 2.
 extern "C" int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
-  const int G = h->H + 1;
+  const int G = h->H + 3;
   if (P) *P = G;
   if (out && h->d_prof) {
     std::vector<unsigned long long> v(8 * G);
