@@ -9,13 +9,14 @@
 #define PS_WAIT_TICKS 50000000ull  // 0.5 s of the 100 MHz wall clock
 
 // What a helper needs of a decided block: the a accepted bonds and the
-// 4a x 4a matrix B with  G_after[:, q] = G_before[:, q] + G_before[:, S_A] B w[q],
-// w[q] = (I - G_before)[S_A, q].  The accept count itself travels in the
-// block's flag word.
+// 4a x 4a factors A (block upper) and T^-1 (block lower, identity diagonal
+// blocks) of B = A T^-1, with
+//     G_after[:, q] = G_before[:, q] + G_before[:, S_A] B w[q],  w[q] = (I - G_before)[S_A, q].
+// The accept count itself travels in the block's flag word.
 struct ps_rec {
   int li[PS_NB];          // block-local indices of the accepted bonds, ascending
   int pad[8];
-  zd B[PS_BS * PS_BS];    // row-major with stride 4a
+  zd AT[PS_BS * PS_BS];   // A on and above the diagonal blocks, T^-1 below; row-major, stride 4a
 };
 
 struct ps_args {

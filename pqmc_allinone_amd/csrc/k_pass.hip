@@ -14,23 +14,26 @@
 // collects the sequence:
 //     A[:, m] = (sel_m + sum_{m'<m} A[:, m'] v_{m'}[S_m]) X_m   (block upper),
 //     T[m][m'] = u_{m'}[S_m] for m' < m, identity blocks on the diagonal.
-// The bonds are cut into decision blocks of nb consecutive bonds; one leader
-// workgroup decides them and H helper workgroups own the columns of G:
+// T^-1 is block lower with identity diagonal blocks, so A and T^-1 share one
+// 4a x 4a array: A in the upper part with the diagonal blocks, T^-1 in the
+// strictly lower part.  The bonds are cut into decision blocks of nb
+// consecutive bonds; one leader workgroup decides them and H helper
+// workgroups own the columns of G:
 //
 //  * Leader, wave 0: decides the bonds of block k from an LDS copy
 //    W = G[S_k, S_k] that it keeps current with the rank-4 updates (only the
 //    rows and columns of the block's later bonds change).
-//  * Leader, wave 1: builds B of block k accept by accept while wave 0 goes
-//    on, and publishes it with the accepted indices.
-//  * Leader, waves 2-7: prepare the proposals of block k+1 and fetch
-//    G[S_{k+1}, S_k], G[S_k, S_{k+1}], G[S_{k+1}, S_{k+1}] after block k-1.
-//    After block k the whole workgroup folds block k into the last tile,
-//        G[S_{k+1}, S_{k+1}] -= (G[S_{k+1}, S_A] B) G[S_A, S_{k+1}],
-//    which is the next window.  The helpers therefore lag by one block.
+//  * Leader, wave 1: extends A and T^-1 by each accept while wave 0 goes on,
+//    and publishes them with the accepted indices.
+//  * Leader, waves 2-5: fetch G[S_{k+1}, S_k], G[S_k, S_{k+1}] and
+//    G[S_{k+1}, S_{k+1}] after block k-1 and, accept by accept, the factors
+//    P = G[S_{k+1}, S_A] A and Q = T^-1 G[S_A, S_{k+1}]; after block k they
+//    fold it into the next window,  G[S_{k+1}, S_{k+1}] -= P Q.  The helpers
+//    therefore lag by one block.  Wave 7 prepares the next block's proposals.
 //  * Helper h keeps the columns of bph bonds in LDS for the whole pass and
-//    applies block k to them in one step, G[:, q] += G[:, S_A] (B w[q]), with
-//    the pivot columns G[:, S_A] read from a slot that the owners of block
-//    k's columns filled after step k-1.
+//    applies block k to them in one step, G[:, q] += G[:, S_A] (B w[q]) with
+//    B = A T^-1 formed locally, and the pivot columns G[:, S_A] read from a
+//    slot that the owners of block k's columns filled after step k-1.
 //
 // Accept test: the reference rejects when prob < u with prob =
 // min(1, exp(min(10, w))), w = dS + ncopy*(1+TR)*log|det(I + G_SS Delta)|
@@ -312,20 +315,46 @@ struct ps_leader_lds {
   zd sU[PS_NB][PS_BS * 4];      // u_m at the block's rows (row*4 + comp)
   zd sV[PS_NB][4 * PS_BS];      // v_m at the block's cols (comp*BS + col)
   zd sX[PS_NB][16];
-  zd sA[PS_BS * PS_BS];         // A (row-major, stride PS_BS); the fold's G[S_{k+1}, S_A] B afterwards
-  zd sTi[PS_BS * PS_BS];        // T^-1 (row-major, stride PS_BS)
-  zd sB[PS_BS * PS_BS];         // B = A T^-1 (row-major, stride PS_BS)
+  zd sAT[PS_BS * PS_BS];        // A (upper part with diagonal blocks) and T^-1 (strictly lower), stride PS_BS
+  zd sPt[PS_BS * PS_BS];        // P transposed: sPt[g * PS_BS + r] = P[r][g]
+  zd sQ[PS_BS * PS_BS];         // Q (row-major, stride PS_BS)
   zd sTt[PS_BS * 4];
   zd sM[4][16], sCof[16];
   zd dets[PS_NB];
   ps_prop prop[2][PS_NB];
   int li[PS_NB];
-  int npub, done, nacc, abort, fold_ctr;
+  int npub, done;    // wave 0 -> wave 1: accepts handed over, block decided
+  int nfold, bdone;  // wave 1 -> waves 2-5: accepts folded into A and T^-1, block folded
+  int tiles;         // waves 2-5 that have stored their part of the fold tiles
+  int nacc, abort;
 };
 
-// Barrier among the n waves that call it, through an LDS counter that the
-// caller resets between uses (lane 0 of each wave counts and waits).
-static __device__ void ps_lds_bar(const ps_args &A, int *ctr, int n) {
+// One wave waits until the LDS count *cnt exceeds have, or until *fin is
+// set; returns the count then seen (have or less means no more will come),
+// or -1 after the clock bound.  The producer sets *fin after its last count.
+static __device__ int ps_lds_progress(const ps_args &A, ps_leader_lds &L, int *cnt, int *fin, int have) {
+  const unsigned long long t0 = wall_clock64();
+  for (unsigned it = 1;; ++it) {
+    const int np = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (np > have) return np;
+    if (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // the producers never wait inside a block; the bound only guards against a logic error
+    if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
+      if ((threadIdx.x & 63) == 0) {
+        __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(&A.stats[2], 1ull);
+        L.abort = 1;
+      }
+      return -1;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Barrier among the n waves that call it, through an LDS counter that is
+// reset between uses (lane 0 of each wave counts and waits).
+static __device__ void ps_lds_bar(const ps_args &A, ps_leader_lds &L, int *ctr, int n) {
   if ((threadIdx.x & 63) == 0) {
     __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     const unsigned long long t0 = wall_clock64();
@@ -333,6 +362,7 @@ static __device__ void ps_lds_bar(const ps_args &A, int *ctr, int n) {
       if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
         __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&A.stats[2], 1ull);
+        L.abort = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -342,17 +372,16 @@ static __device__ void ps_lds_bar(const ps_args &A, int *ctr, int n) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Wave 1: fold accept m of the current block into A, T^-1 and B.
+// Wave 1: extend A and T^-1 by accept m of the current block.
 static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
   const int n = 4 * (m + 1);
   const int jm = L.li[m];
-  zd *__restrict__ sA = L.sA;
-  zd *__restrict__ sTi = L.sTi;
-  zd *__restrict__ sB = L.sB;
+  zd *__restrict__ sAT = L.sAT;
   zd *__restrict__ sTt = L.sTt;
   // T(g, kk) = sel + sum_{m'<m} A[g][4m'+d] v_{m'}[d][S_m(kk)]  (A[g][4m'+d] = 0 unless g/4 <= m')
-  // T^-1[4m + r][g] = sel - sum_{m'<m} u_{m'}[S_m(r)][d] T^-1[4m'+d][g]  (zero unless g/4 <= m')
-  for (int e = lane; e < 8 * n; e += 64) {
+  // T^-1[4m + r][g] = - sum_{m'<m} u_{m'}[S_m(r)][d] T^-1[4m'+d][g]  for g < 4m, where
+  // T^-1[4m'+d][g] is the identity on the diagonal block m' = g/4 and stored below it
+  for (int e = lane; e < 4 * n + 4 * 4 * m; e += 64) {
     if (e < 4 * n) {
       const int g = e >> 2, kk = e & 3;
       zd t = zmk(g == 4 * m + kk ? 1.0 : 0.0, 0.0);
@@ -360,7 +389,7 @@ static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
         zd av[4], vv[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-          av[d] = sA[g * PS_BS + 4 * mp + d];
+          av[d] = sAT[g * PS_BS + 4 * mp + d];
           vv[d] = L.sV[mp][d * PS_BS + 4 * jm + kk];
         }
 #pragma unroll
@@ -368,20 +397,22 @@ static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
       }
       sTt[g * 4 + kk] = t;
     } else {
-      const int e2 = e - 4 * n;
-      const int r = e2 / n, g = e2 % n;
-      zd t = zmk(g == 4 * m + r ? 1.0 : 0.0, 0.0);
-      for (int mp = g >> 2; mp < m; ++mp) {
+      const int e2 = e - 4 * n, nl = 4 * m;
+      const int r = e2 / nl, g = e2 % nl;
+      const int gb = g >> 2;
+      // diagonal block m' = g/4: -u_{m'}[S_m(r)][g % 4]
+      zd t = zneg(L.sU[gb][(4 * jm + r) * 4 + (g & 3)]);
+      for (int mp = gb + 1; mp < m; ++mp) {
         zd uv[4], tv[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
           uv[d] = L.sU[mp][(4 * jm + r) * 4 + d];
-          tv[d] = sTi[(4 * mp + d) * PS_BS + g];
+          tv[d] = sAT[(4 * mp + d) * PS_BS + g];
         }
 #pragma unroll
         for (int d = 0; d < 4; ++d) t = zsub(t, zmul(uv[d], tv[d]));
       }
-      sTi[(4 * m + r) * PS_BS + g] = t;
+      sAT[(4 * m + r) * PS_BS + g] = t;
     }
   }
   wave_sync();
@@ -397,24 +428,47 @@ static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
     zd s = zmk(0.0, 0.0);
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) s = zadd(s, zmul(tv[kk], xv[kk]));
-    sA[g * PS_BS + 4 * m + c] = s;
+    sAT[g * PS_BS + 4 * m + c] = s;
   }
   wave_sync();
-  // B += A[:, block m] T^-1[block m, :]
-  for (int e = lane; e < n * n; e += 64) {
-    const int g = e / n, h = e % n;
-    zd av[4], tv[4];
+}
+
+// Waves 2-5 (t2 = 0..255): the contributions of accept m to
+//     P[:, block m] = G[S_{k+1}, S_A] A[:, block m]      (t2 < 128: row t2/4, column t2%4)
+//     Q[block m, :] = T^-1[block m, :] G[S_A, S_{k+1}]   (t2 >= 128: row (t2-128)/32, column t2%32)
+// with T10 stored transposed.  Entries are independent, so no barrier is needed.
+static __device__ void ps_pq_accept(ps_leader_lds &L, int m, int t2, const zd *T10t, const zd *T01) {
+  if (t2 < 128) {
+    const int r = t2 >> 2, c = t2 & 3;
+    zd s0 = zmk(0.0, 0.0), s1 = s0;
+    for (int mb = 0; mb <= m; ++mb) {
+      const int cb = 4 * L.li[mb];
+      zd tv[4], av[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      av[c] = sA[g * PS_BS + 4 * m + c];
-      tv[c] = sTi[(4 * m + c) * PS_BS + h];
+      for (int d = 0; d < 4; ++d) {
+        tv[d] = T10t[(cb + d) * PS_BS + r];
+        av[d] = L.sAT[(4 * mb + d) * PS_BS + 4 * m + c];
+      }
+      s0 = zadd(s0, zadd(zmul(tv[0], av[0]), zmul(tv[1], av[1])));
+      s1 = zadd(s1, zadd(zmul(tv[2], av[2]), zmul(tv[3], av[3])));
     }
-    zd s = sB[g * PS_BS + h];
+    L.sPt[(4 * m + c) * PS_BS + r] = zadd(s0, s1);
+  } else {
+    const int e = t2 - 128, rr = e >> 5, c = e & 31;
+    zd s0 = T01[(4 * L.li[m] + rr) * PS_BS + c], s1 = zmk(0.0, 0.0);
+    for (int mb = 0; mb < m; ++mb) {
+      const int cb = 4 * L.li[mb];
+      zd tv[4], ov[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s = zadd(s, zmul(av[c], tv[c]));
-    sB[g * PS_BS + h] = s;
+      for (int d = 0; d < 4; ++d) {
+        tv[d] = L.sAT[(4 * m + rr) * PS_BS + 4 * mb + d];
+        ov[d] = T01[(cb + d) * PS_BS + c];
+      }
+      s0 = zadd(s0, zadd(zmul(tv[0], ov[0]), zmul(tv[1], ov[1])));
+      s1 = zadd(s1, zadd(zmul(tv[2], ov[2]), zmul(tv[3], ov[3])));
+    }
+    L.sQ[(4 * m + rr) * PS_BS + c] = zadd(s0, s1);
   }
-  wave_sync();
 }
 
 static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
@@ -431,7 +485,9 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     L.abort = 0;
     L.npub = 0;
     L.done = 0;
-    L.fold_ctr = 0;
+    L.nfold = 0;
+    L.bdone = 0;
+    L.tiles = 0;
     L.nacc = ps_wait(A, ps_arr(A, -1), (unsigned long long)A.H, nullptr, 0) ? 1 : 0;
   }
   __syncthreads();
@@ -591,49 +647,46 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       ps_release();  // acc flags, fields and exponentials are complete before block k is published
       PROF_ACC(0);
     } else if (wave == 1) {
-      // ---- B of block k, accept by accept ----
-      for (int e = lane; e < PS_BS * PS_BS; e += 64) L.sB[e] = zmk(0.0, 0.0);
-      wave_sync();
-      PROF_ACC(3);
+      // ---- A and T^-1 of block k, accept by accept ----
       int md_ = 0;
-      const unsigned long long t0 = wall_clock64();
-      for (unsigned it = 1;; ++it) {
-        // wave 0 never waits inside a block, so this ends once it is done;
-        // the clock bound only guards against a logic error
-        int np = __hip_atomic_load(&L.npub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (np <= md_) {
-          if (__hip_atomic_load(&L.done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-            np = __hip_atomic_load(&L.npub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (np <= md_) break;
-          } else {
-            if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
-              if (lane == 0) {
-                __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                atomicAdd(&A.stats[2], 1ull);
-                L.abort = 1;
-              }
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-        }
+      for (;;) {
         PROF_ACC(0);
-        for (; md_ < np; ++md_) ps_fold_accept(L, md_, lane);
+        const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
+        PROF_ACC(0);
+        if (np <= md_) break;
+        for (; md_ < np; ++md_) {
+          ps_fold_accept(L, md_, lane);
+          if (lane == 0) __hip_atomic_store(&L.nfold, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
         PROF_ACC(1);
       }
-      PROF_ACC(0);
+      if (lane == 0) __hip_atomic_store(&L.bdone, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // the record's data now; its flag after the barrier
+      const int a = L.nacc, n4 = 4 * a;
+      ps_rec *R = A.rec + k;
+      if (lane < a) st_coh_i(&R->li[lane], L.li[lane]);
+      for (int e = lane; e < n4 * n4; e += 64) st_coh(&R->AT[e], L.sAT[(e / n4) * PS_BS + e % n4]);
+      PROF_ACC(4);
     } else if (has_next && wave <= 5) {
-      // ---- waves 2..5: the fold tiles after step k-1 ----
+      // ---- waves 2..5: the fold tiles after step k-1, then P and Q ----
       bool wok = true;
       if (lane == 0) wok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
       wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
       PROF_ACC(1);
-      if (wok)
+      if (wok) {
         ps_load_fold_tiles(A, k, L.buf[i10], L.buf[i01], L.buf[i11], tid - 128);
-      else if (lane == 0)
+        ps_lds_bar(A, L, &L.tiles, 4);  // P and Q read all of the tiles
+        PROF_ACC(7);
+        int md_ = 0;
+        for (;;) {
+          const int np = ps_lds_progress(A, L, &L.nfold, &L.bdone, md_);
+          if (np <= md_) break;
+          for (; md_ < np; ++md_) ps_pq_accept(L, md_, tid - 128, L.buf[i10], L.buf[i01]);
+        }
+      } else if (lane == 0) {
         L.abort = 1;
-      PROF_ACC(7);
+      }
+      PROF_ACC(5);
     } else if (has_next && wave == 7) {
       // ---- wave 7: the next block's proposals ----
       const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
@@ -643,12 +696,9 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     __syncthreads();
     if (L.abort) { ok = false; break; }
     PROF_ACC(2);
-    const int a = L.nacc, n4 = 4 * a;
+    const int a = L.nacc;
     if (wave == 1) {
-      // ---- publish block k: accepted indices, B, then the flag ----
-      ps_rec *R = A.rec + k;
-      if (lane < a) st_coh_i(&R->li[lane], L.li[lane]);
-      for (int e = lane; e < n4 * n4; e += 64) st_coh(&R->B[e], L.sB[(e / n4) * PS_BS + e % n4]);
+      // ---- publish block k: the record stores are complete, then the flag ----
       ps_release();
       if (lane == 0) ps_st(ps_flag(A, k), (unsigned long long)(a + 1));
       // ln_cw += log(real(R_elec)) per accept (src/mod_update.f90:142)
@@ -665,40 +715,14 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       PROF_ACC(4);
     }
     if (!has_next) break;
-    const int nn = ps_nsites(A, k + 1);
-    zd *T10 = L.buf[i10], *T01 = L.buf[i01], *T11 = L.buf[i11];
-    if (a > 0 && (wave == 0 || (wave >= 2 && wave <= 4))) {
-      // ---- fold block k into the next window: T11 -= (T10[:, S_A] B) T01[S_A, :] ----
-      // on FP64 MFMA, one 16 x 16 tile per wave (waves 0, 2, 3, 4) while wave 1
-      // publishes; lane l supplies A[l%16][l/16] and B[l/16][l%16], and holds
-      // rows l/16 + 4i of column l%16 of the result.
-      const int tt = wave == 0 ? 0 : wave - 1;
+    zd *T11 = L.buf[i11];
+    if (a > 0 && wave >= 2 && wave <= 5) {
+      // ---- fold block k into the next window: T11 -= P Q, K = 4a ----
+      // on FP64 MFMA, one 16 x 16 tile per wave; lane l supplies A[l%16][l/16]
+      // and B[l/16][l%16], and holds rows l/16 + 4i of column l%16 of the result.
+      const int tt = wave - 2;
       const int rt = tt >> 1, ct = tt & 1;
       const int lr = lane & 15, lk = lane >> 4;
-      zd *C = L.sA;
-      if (16 * ct < n4) {
-        // two k-steps per round into two interleaved MFMA chains
-        double4_t cre0 = {0.0, 0.0, 0.0, 0.0}, cim0 = cre0, cre1 = cre0, cim1 = cre0;
-        for (int ks = 0; ks < a; ks += 2) {
-          const int k2 = min(ks + 1, a - 1);
-          const zd a0 = T10[(4 * L.li[ks] + lk) * PS_BS + 16 * rt + lr];  // T10 is stored transposed
-          const zd b0 = L.sB[(4 * ks + lk) * PS_BS + 16 * ct + lr];
-          const zd a1 = T10[(4 * L.li[k2] + lk) * PS_BS + 16 * rt + lr];
-          const zd b1 = zsel(ks + 1 < a, L.sB[(4 * k2 + lk) * PS_BS + 16 * ct + lr]);
-          cre0 = mfma_f64(a0.x, b0.x, cre0);
-          cim0 = mfma_f64(a0.x, b0.y, cim0);
-          cre1 = mfma_f64(a1.x, b1.x, cre1);
-          cim1 = mfma_f64(a1.x, b1.y, cim1);
-          cre0 = mfma_f64(-a0.y, b0.y, cre0);
-          cim0 = mfma_f64(a0.y, b0.x, cim0);
-          cre1 = mfma_f64(-a1.y, b1.y, cre1);
-          cim1 = mfma_f64(a1.y, b1.x, cim1);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          C[(16 * ct + lr) * PS_BS + 16 * rt + lk + 4 * i] = zmk(cre0[i] + cre1[i], cim0[i] + cim1[i]);  // transposed
-      }
-      ps_lds_bar(A, &L.fold_ctr, 4);
       double4_t dre0, dim0, dre1 = {0.0, 0.0, 0.0, 0.0}, dim1 = dre1;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -708,10 +732,10 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       }
       for (int ks = 0; ks < a; ks += 2) {
         const int k2 = min(ks + 1, a - 1);
-        const zd a0 = C[(4 * ks + lk) * PS_BS + 16 * rt + lr];
-        const zd b0 = T01[(4 * L.li[ks] + lk) * PS_BS + 16 * ct + lr];
-        const zd a1 = zsel(ks + 1 < a, C[(4 * k2 + lk) * PS_BS + 16 * rt + lr]);
-        const zd b1 = T01[(4 * L.li[k2] + lk) * PS_BS + 16 * ct + lr];
+        const zd a0 = L.sPt[(4 * ks + lk) * PS_BS + 16 * rt + lr];
+        const zd b0 = L.sQ[(4 * ks + lk) * PS_BS + 16 * ct + lr];
+        const zd a1 = zsel(ks + 1 < a, L.sPt[(4 * k2 + lk) * PS_BS + 16 * rt + lr]);
+        const zd b1 = L.sQ[(4 * k2 + lk) * PS_BS + 16 * ct + lr];
         dre0 = mfma_f64(-a0.x, b0.x, dre0);
         dim0 = mfma_f64(-a0.x, b0.y, dim0);
         dre1 = mfma_f64(-a1.x, b1.x, dre1);
@@ -732,9 +756,11 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     if (tid == 0) {
       L.npub = 0;
       L.done = 0;
+      L.nfold = 0;
+      L.bdone = 0;
+      L.tiles = 0;
     }
     __syncthreads();
-    if (tid == 0) L.fold_ctr = 0;  // all fold waves have left their LDS barrier
     // the window of block k+1 is T11; the old window's buffer becomes T11
     const int tmp = iW;
     iW = i11;
@@ -752,7 +778,8 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
 // Helpers: own columns, wraps, block steps.
 // --------------------------------------------------------------------------
 struct ps_helper_hdr {
-  zd sB[PS_BS * PS_BS];        // B (row-major, stride PS_BS)
+  zd sAT[PS_BS * PS_BS];       // A and T^-1 of the block (packed as in the record, stride PS_BS)
+  zd sB[PS_BS * PS_BS];        // B = A T^-1 (row-major, stride PS_BS)
   int li[PS_NB];
   int a, ok, sink;
 };
@@ -912,15 +939,23 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     if (a > 0) {
       const ps_rec *R = A.rec + k;
       if (tid < a) Hh.li[tid] = R->li[tid];
-      for (int e = tid; e < n4 * n4; e += PS_THREADS) Hh.sB[(e / n4) * PS_BS + e % n4] = R->B[e];
+      for (int e = tid; e < n4 * n4; e += PS_THREADS) Hh.sAT[(e / n4) * PS_BS + e % n4] = R->AT[e];
       __syncthreads();
       PROF_ACC(2);
-      // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q]
+      // w[q][g] = e_q(S_g) - G_k[S_g, q] and B = A T^-1:
+      //   B[g][h] = A[g][h] (h/4 >= g/4) + sum_{j/4 > h/4, j/4 >= g/4} A[g][j] T^-1[j][h]
       const int *Sk = sS + 4 * A.nb * k;
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
         const int q = e / n4, g = e % n4;
         const int st = Sk[4 * Hh.li[g >> 2] + (g & 3)];
         sw[q * PS_BS + g] = zsub(zmk(sS[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
+      }
+      for (int e = tid; e < n4 * n4; e += PS_THREADS) {
+        const int g = e / n4, h = e % n4;
+        const int gb = g >> 2, hb = h >> 2;
+        zd b = hb >= gb ? Hh.sAT[g * PS_BS + h] : zmk(0.0, 0.0);
+        for (int j = 4 * max(gb, hb + 1); j < n4; ++j) b = zfma(b, Hh.sAT[g * PS_BS + j], Hh.sAT[j * PS_BS + h]);
+        Hh.sB[g * PS_BS + h] = b;
       }
       __syncthreads();
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
