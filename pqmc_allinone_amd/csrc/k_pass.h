@@ -32,9 +32,11 @@ struct ps_args {
   zd *G;              // Ns x Ns, column-major, in/out
   zd *slot;           // [nblk + 1][2][PS_BS][lds] columns of blocks k+1, k+2 after step k = -1..nblk-1
   int lds;            // column stride of the slot: Ns rounded up to whole 128-byte lines
+  zd *tslot;          // [nblk + 1][2][PS_BS][2 PS_BS] the same columns at the rows of blocks k+1, k+2
   ps_rec *rec;        // [nblk]
-  unsigned long long *sync;  // [1] done, [2] abort, [8 + k + 1] arrivals after step k (k = -1..nblk-1),
-                             // [9 + nblk + k] block k published: accepts + 1
+  unsigned long long *sync;  // [1] done, [2] abort, [8 + 2(k + 1) + mb] owners of block k+1+mb that have
+                             // published after step k (k = -1..nblk-1), [10 + 2 nblk + k] block k
+                             // decided: accepts + 1
   uint8_t *acc;       // [npla]
   double *lncw_acc;
   unsigned long long *stats;  // [0] accepted, [1] rejected, [2] wait timeouts
@@ -46,8 +48,9 @@ static inline int pqmc_pass_slot_ld(int Ns) { return (Ns + 7) & ~7; }
 static inline size_t pqmc_pass_slot_elems(int Ns, int nblk) {
   return (size_t)(nblk + 1) * 2 * PS_BS * pqmc_pass_slot_ld(Ns);
 }
+static inline size_t pqmc_pass_tslot_elems(int nblk) { return (size_t)(nblk + 1) * 2 * PS_BS * 2 * PS_BS; }
 // words of the sync array for nblk decision blocks
-static inline size_t pqmc_pass_sync_words(int nblk) { return 16 + 2 * (size_t)nblk; }
+static inline size_t pqmc_pass_sync_words(int nblk) { return 16 + 3 * (size_t)nblk; }
 
 extern "C" {
 size_t pqmc_pass_lds(int Ns, int bph);

@@ -135,9 +135,21 @@ static __device__ unsigned long long ps_wait(const ps_args &A, unsigned long lon
   }
 }
 
-static __device__ __forceinline__ unsigned long long *ps_arr(const ps_args &A, int k) { return &A.sync[8 + k + 1]; }
+// count of the owners of block kstep+1+mb that have published after step kstep
+static __device__ __forceinline__ unsigned long long *ps_pub(const ps_args &A, int kstep, int mb) {
+  return &A.sync[8 + 2 * (kstep + 1) + mb];
+}
 static __device__ __forceinline__ unsigned long long *ps_flag(const ps_args &A, int k) {
-  return &A.sync[9 + A.nblk + k];
+  return &A.sync[10 + 2 * A.nblk + k];
+}
+// helpers that own at least one bond of block blk
+static __device__ __forceinline__ unsigned long long ps_owners(const ps_args &A, int blk) {
+  const int b0 = blk * A.nb, b1 = min((blk + 1) * A.nb, A.md.npla) - 1;
+  return (unsigned long long)(b1 / A.bph - b0 / A.bph + 1);
+}
+// column cl (of 2 PS_BS rows: block kstep+1, then block kstep+2) of block kstep+1+mb in the tile slot
+static __device__ __forceinline__ zd *ps_tcol(const ps_args &A, int kstep, int mb, int cl) {
+  return A.tslot + ((size_t)(((kstep + 1) * 2 + mb) * PS_BS + cl)) * (2 * PS_BS);
 }
 // column cl of block kstep+1+mb in the slot written after step kstep
 static __device__ __forceinline__ zd *ps_scol(const ps_args &A, int kstep, int mb, int cl) {
@@ -261,46 +273,27 @@ static __device__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
     }
 }
 
-// Gather a tile G[rows of block rb, cols of block cb] from the slot written
-// after step kstep (blocks kstep+1, kstep+2).  T is row-major with stride PS_BS.
-static __device__ void ps_load_tile(const ps_args &A, int kstep, int rb, int cb, zd *T, int t0, int nt) {
-  const int nr = ps_nsites(A, rb), nc = ps_nsites(A, cb);
-  const int *rs = A.fsites + 4 * A.nb * rb;
-  for (int e = t0; e < PS_BS * PS_BS; e += nt) {
-    const int r = e / PS_BS, c = e % PS_BS;
-    zd v = zmk(0.0, 0.0);
-    if (r < nr && c < nc) v = ps_scol(A, kstep, cb - (kstep + 1), c)[rs[r]];
-    T[e] = v;
-  }
-}
-
-// The three tiles that fold block k into the window of block k+1, from the
-// slot written after step k-1: T10 = G[S_{k+1}, S_k] stored transposed
+// The three tiles that fold block k into the window of block k+1, as they
+// were after step k-1: T10 = G[S_{k+1}, S_k] stored transposed
 // (T10t[c * PS_BS + r]), T01 = G[S_k, S_{k+1}] and T11 = G[S_{k+1}, S_{k+1}]
 // row-major; entries outside the blocks are zero.  256 threads (t0 = 0..255)
 // issue all their loads before the first LDS store.
 static __device__ void ps_load_fold_tiles(const ps_args &A, int k, zd *T10t, zd *T01, zd *T11, int t0) {
-  // thread t0 takes column c = t0 % 32 and rows r0 + 8j (j = 0..3) of each tile
+  // thread t0 takes row r = t0 % 32 and columns c0 + 8j (j = 0..3) of each tile,
+  // from the compact tile slot written after step k-1 (rows: block k, then k+1)
   const int n0 = ps_nsites(A, k), n1 = ps_nsites(A, k + 1);
-  const int c = t0 & 31, r0 = t0 >> 5;
-  const int *S0 = A.fsites + 4 * A.nb * k, *S1 = S0 + 4 * A.nb;
-  const zd *col0 = ps_scol(A, k - 1, 0, c), *col1 = ps_scol(A, k - 1, 1, c);
-  int s0[4], s1[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    s0[j] = S0[min(r0 + 8 * j, n0 - 1)];
-    s1[j] = S1[min(r0 + 8 * j, n1 - 1)];
-  }
+  const int r = t0 & 31, c0 = t0 >> 5;
   zd v10[4], v01[4], v11[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    v10[j] = col0[s1[j]];  // G[S_{k+1}(r), S_k(c)]
-    v01[j] = col1[s0[j]];  // G[S_k(r), S_{k+1}(c)]
-    v11[j] = col1[s1[j]];  // G[S_{k+1}(r), S_{k+1}(c)]
+    const int c = c0 + 8 * j;
+    v10[j] = ps_tcol(A, k - 1, 0, c)[PS_BS + r];  // G[S_{k+1}(r), S_k(c)]
+    v01[j] = ps_tcol(A, k - 1, 1, c)[r];          // G[S_k(r), S_{k+1}(c)]
+    v11[j] = ps_tcol(A, k - 1, 1, c)[PS_BS + r];  // G[S_{k+1}(r), S_{k+1}(c)]
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int r = r0 + 8 * j;
+    const int c = c0 + 8 * j;
     T10t[c * PS_BS + r] = zsel(r < n1 && c < n0, v10[j]);
     T01[r * PS_BS + c] = zsel(r < n0 && c < n1, v01[j]);
     T11[r * PS_BS + c] = zsel(r < n1 && c < n1, v11[j]);
@@ -488,11 +481,17 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     L.nfold = 0;
     L.bdone = 0;
     L.tiles = 0;
-    L.nacc = ps_wait(A, ps_arr(A, -1), (unsigned long long)A.H, nullptr, 0) ? 1 : 0;
+    L.nacc = ps_wait(A, ps_pub(A, -1, 0), ps_owners(A, 0), nullptr, 0) ? 1 : 0;
   }
   __syncthreads();
   ok = L.nacc != 0;
-  if (ok) ps_load_tile(A, -1, 0, 0, L.buf[iW], tid, PS_THREADS);
+  if (ok) {
+    const int n0 = ps_nsites(A, 0);
+    for (int e = tid; e < PS_BS * PS_BS; e += PS_THREADS) {
+      const int c = e >> 5, r = e & 31;
+      L.buf[iW][r * PS_BS + c] = zsel(r < n0 && c < n0, ps_tcol(A, -1, 0, c)[r]);
+    }
+  }
   __syncthreads();
   PROF_ACC(0);
   int total_acc = 0, total_rej = 0;
@@ -670,7 +669,8 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     } else if (has_next && wave <= 5) {
       // ---- waves 2..5: the fold tiles after step k-1, then P and Q ----
       bool wok = true;
-      if (lane == 0) wok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
+      if (lane == 0)
+        wok = ps_wait(A, ps_pub(A, k - 1, 0), ps_owners(A, k), ps_pub(A, k - 1, 1), ps_owners(A, k + 1)) != 0;
       wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
       PROF_ACC(1);
       if (wok) {
@@ -821,19 +821,37 @@ static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bo
   __syncthreads();
 }
 
-static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, int b0, int b1, int kstep) {
-  // after step kstep the slot holds the columns of blocks kstep+1, kstep+2
+// After step kstep the slot holds the columns of blocks kstep+1, kstep+2 and
+// the tile slot the same columns at the rows of those two blocks.
+static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, const int *sS, int b0, int b1, int kstep) {
   const int Ns = A.md.Ns, tid = threadIdx.x;
   for (int b = b0; b < b1; ++b) {
     const int blk = b / A.nb;
     const int mb = blk - (kstep + 1);
     if (mb < 0 || mb > 1) continue;
     const int cl0 = 4 * (b - blk * A.nb);
+    const zd *col = sC + 4 * (b - b0) * Ns;
     for (int e = tid; e < 4 * Ns; e += PS_THREADS) {
       const int c = e / Ns, r = e % Ns;
-      st_coh(ps_scol(A, kstep, mb, cl0 + c) + r, sC[(4 * (b - b0) + c) * Ns + r]);
+      st_coh(ps_scol(A, kstep, mb, cl0 + c) + r, col[c * Ns + r]);
+    }
+    for (int e = tid; e < 4 * 2 * PS_BS; e += PS_THREADS) {
+      const int c = e >> 6, i = e & 63, j = kstep + 1 + (i >> 5), r = i & 31;
+      if (j < A.nblk && r < ps_nsites(A, j)) st_coh(ps_tcol(A, kstep, mb, cl0 + c) + i, col[c * Ns + sS[4 * A.nb * j + r]]);
     }
   }
+}
+
+// One thread: count this helper among the publishers of the blocks
+// kstep+1, kstep+2 that it owns bonds of (after its stores are complete).
+static __device__ void ps_announce(const ps_args &A, int b0, int b1, int kstep) {
+  bool own[2] = {false, false};
+  for (int b = b0; b < b1; ++b) {
+    const int mb = b / A.nb - (kstep + 1);
+    if (mb == 0 || mb == 1) own[mb] = true;
+  }
+  for (int mb = 0; mb < 2; ++mb)
+    if (own[mb]) atomicAdd(ps_pub(A, kstep, mb), 1ull);
 }
 
 // G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own columns, four at a
@@ -903,17 +921,18 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
   for (int e = tid; e < 4 * npla; e += PS_THREADS) sS[e] = A.fsites[e];
   __syncthreads();
   if (A.wrap == 2) ps_wrap_cols(A, sC, b0, b1, false);
-  ps_publish_cols(A, sC, b0, b1, -1);
+  ps_publish_cols(A, sC, sS, b0, b1, -1);
   ps_release();
   __syncthreads();
-  if (tid == 0) atomicAdd(ps_arr(A, -1), 1ull);
+  if (tid == 0) ps_announce(A, b0, b1, -1);
   PROF_ACC(0);
   for (int k = 0; k < A.nblk; ++k) {
-    // Block k's columns are in the slot once step k-1 is complete; the
-    // leader's decision of block k comes later.  Meanwhile the helpers of an
-    // XCD (workgroups are spread over the 8 XCDs round-robin) touch those
-    // columns in slices, so that the pivot loads of step k hit in L2.
-    if (tid == 0) Hh.ok = ps_wait(A, ps_arr(A, k - 1), (unsigned long long)A.H, nullptr, 0) != 0;
+    // Block k's columns are in the slot once their owners have published
+    // after step k-1; the leader's decision of block k comes later.
+    // Meanwhile the helpers of an XCD (workgroups are spread over the 8 XCDs
+    // round-robin) touch those columns in slices, so that the pivot loads of
+    // step k hit in L2.
+    if (tid == 0) Hh.ok = ps_wait(A, ps_pub(A, k - 1, 0), ps_owners(A, k), nullptr, 0) != 0;
     __syncthreads();
     if (!Hh.ok) break;
     {
@@ -953,14 +972,23 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
       for (int e = tid; e < n4 * n4; e += PS_THREADS) {
         const int g = e / n4, h = e % n4;
         const int gb = g >> 2, hb = h >> 2;
-        zd b = hb >= gb ? Hh.sAT[g * PS_BS + h] : zmk(0.0, 0.0);
-        for (int j = 4 * max(gb, hb + 1); j < n4; ++j) b = zfma(b, Hh.sAT[g * PS_BS + j], Hh.sAT[j * PS_BS + h]);
-        Hh.sB[g * PS_BS + h] = b;
+        zd b0 = hb >= gb ? Hh.sAT[g * PS_BS + h] : zmk(0.0, 0.0), b1 = zmk(0.0, 0.0);
+        for (int j = 4 * max(gb, hb + 1); j < n4; j += 4) {
+          zd av[4], tv[4];
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            av[d] = Hh.sAT[g * PS_BS + j + d];
+            tv[d] = Hh.sAT[(j + d) * PS_BS + h];
+          }
+          b0 = zfma(zfma(b0, av[0], tv[0]), av[1], tv[1]);
+          b1 = zfma(zfma(b1, av[2], tv[2]), av[3], tv[3]);
+        }
+        Hh.sB[g * PS_BS + h] = zadd(b0, b1);
       }
       __syncthreads();
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
         const int q = e / n4, g = e % n4;
-        zd y = zmk(0.0, 0.0);
+        zd y0 = zmk(0.0, 0.0), y1 = y0;
         for (int hh = 0; hh < n4; hh += 4) {
           zd bv[4], wv[4];
 #pragma unroll
@@ -968,10 +996,10 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
             bv[d] = Hh.sB[g * PS_BS + hh + d];
             wv[d] = sw[q * PS_BS + hh + d];
           }
-#pragma unroll
-          for (int d = 0; d < 4; ++d) y = zfma(y, bv[d], wv[d]);
+          y0 = zfma(zfma(y0, bv[0], wv[0]), bv[1], wv[1]);
+          y1 = zfma(zfma(y1, bv[2], wv[2]), bv[3], wv[3]);
         }
-        sy[q * PS_BS + g] = y;
+        sy[q * PS_BS + g] = zadd(y0, y1);
       }
       __syncthreads();
       PROF_ACC(3);
@@ -980,11 +1008,11 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
       __syncthreads();
     }
     PROF_ACC(5);
-    ps_publish_cols(A, sC, b0, b1, k);
+    ps_publish_cols(A, sC, sS, b0, b1, k);
     ps_release();
     __syncthreads();
     PROF_ACC(6);
-    if (tid == 0) atomicAdd(ps_arr(A, k), 1ull);
+    if (tid == 0) ps_announce(A, b0, b1, k);
     PROF_ACC(7);
   }
   if (A.wrap == 1) ps_wrap_cols(A, sC, b0, b1, true);
@@ -1024,7 +1052,10 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     const unsigned long long done = atomicAdd(&A.sync[1], 1ull);
     if (done == gridDim.x - 1) {
       ps_st(&A.sync[2], 0ull);
-      for (int k = -1; k < A.nblk; ++k) ps_st(ps_arr(A, k), 0ull);
+      for (int k = -1; k < A.nblk; ++k) {
+        ps_st(ps_pub(A, k, 0), 0ull);
+        ps_st(ps_pub(A, k, 1), 0ull);
+      }
       for (int k = 0; k < A.nblk; ++k) ps_st(ps_flag(A, k), 0ull);
       ps_st(&A.sync[1], 0ull);
     }

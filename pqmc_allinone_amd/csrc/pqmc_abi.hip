@@ -106,6 +106,7 @@ struct pqmc_handle {
   size_t u_cap = 0;
   uint8_t *d_acc = nullptr;
   zd *d_slot = nullptr;
+  zd *d_tslot = nullptr;
   ps_rec *d_rec = nullptr;
   unsigned long long *d_psync = nullptr;
   double *d_lncw_acc = nullptr;
@@ -249,6 +250,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->qm.vn2, ne);
   A_(h->d_acc, (size_t)h->Nb * h->Lt);
   A_(h->d_slot, pqmc_pass_slot_elems(Ns, h->nblk));
+  A_(h->d_tslot, pqmc_pass_tslot_elems(h->nblk));
   A_(h->d_rec, (size_t)h->nblk);
   A_(h->d_psync, pqmc_pass_sync_words(h->nblk));
   A_(h->d_lncw_acc, 1);
@@ -296,7 +298,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
   void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_scr,
                   h->d_Q, h->d_D, h->d_R, h->d_dlog, h->d_rinv, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
-                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_rec, h->d_psync, h->d_lncw_acc,
+                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -542,6 +544,7 @@ static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u,
   a.G = h->d_G;
   a.slot = h->d_slot;
   a.lds = pqmc_pass_slot_ld(h->Ns);
+  a.tslot = h->d_tslot;
   a.rec = h->d_rec;
   a.sync = h->d_psync;
   a.acc = d_acc;
