@@ -115,11 +115,21 @@ int pqmc_get_g(pqmc_handle *h, double *g);
 int pqmc_set_g(pqmc_handle *h, const double *g);
 int pqmc_get_qstring(pqmc_handle *h, int block, double *q);   /* Q_string(:,:,block) */
 int pqmc_get_dstring(pqmc_handle *h, double *d);              /* D_string(ne, nblock) */
-int pqmc_get_lncw(pqmc_handle *h, double *re_im);              /* ln_cw */
+/* ln_cw: the last stabilization's value plus the accepts' log(real(R_elec))
+ * since then (src/mod_update.f90:142, src/mod_evolution.f90:330-357). */
+int pqmc_get_lncw(pqmc_handle *h, double *re_im);
 int pqmc_get_accepts(pqmc_handle *h, uint8_t *acc, int64_t n); /* last sweep, call order */
 /* positive_accept, negative_accept, sync timeouts; err_fast_max. */
 int pqmc_get_counters(pqmc_handle *h, int64_t *out3, double *err_fast_max);
 int pqmc_synchronize(pqmc_handle *h);
+
+/* Global moves: the full-determinant weight of the field configuration x
+ * (Nb*Lt, boson_field layout), replacing cal_new_weight_T0
+ * (src/mod_evolution.f90:359-408) as called by update_global
+ * (src/mod_update.f90:251-290).  The chain's fields, Green's function and
+ * B-strings are left unchanged; after an accepted move the caller sets the
+ * fields and runs pqmc_init_g(1), as src/Main_PQMC.f90:94 does. */
+int pqmc_global_weight(pqmc_handle *h, const double *x, double *re_im);
 
 /* The ran2 generator of src/mod_ranstate.f90:171-191 seeded as
  * init_rng_by_seed (:267-283); host-side, for drivers without their own. */

@@ -47,6 +47,7 @@ struct orc_model {
   zc ln_cw;
   double err_fast, err_fast_max;
   long pos_acc, neg_acc;
+  long global_acc, global_rej;
   /* measurement registry (input/model/EPSOCZ.nml:262-283) */
   int total_width;
 };
@@ -1138,6 +1139,208 @@ void orc_ran2_fill(orc_ran2 *s, int64_t n, float *out) {
     int32_t sel = (v < 0) ? v : ~v;
     out[q] = amm * (float)sel;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* global moves (src/mod_update.f90:251-467, src/mod_evolution.f90:359-408) */
+/* ------------------------------------------------------------------ */
+
+/* zlq of src/mod_matrixlib.f90:488-503 (ZGELQF + ZUNGLQ): a(d x n) = L Q,
+ * a <- Q, l (d x d) the lower factor.  ZGELQF reflects the conjugated rows
+ * with zlarfg, which is zgeqr2 on a^H: L = R^H with the same real diagonal
+ * and Q = Q'^H.  The diagonal keeps zlarfg's (beta, +0): a negative entry
+ * then has log = log|beta| + i pi as in the reference, not the -i pi that a
+ * conjugated -0 imaginary part would give. */
+static void zlq(int d, int n, zc *a, zc *l) {
+  zc *at = xcalloc((size_t)n * d, sizeof(zc));
+  zc *tau = xcalloc((size_t)d, sizeof(zc));
+  conj_transpose(d, n, a, at);
+  zgeqr2(n, d, at, n, tau);
+  for (int j = 0; j < d; ++j)
+    for (int i = 0; i < d; ++i)
+      l[i + (size_t)d * j] = (i > j) ? conj(at[j + (size_t)n * i]) : (i == j) ? CMPLX(creal(at[j + (size_t)n * j]), 0.0) : 0;
+  zung2r(n, d, at, n, tau);
+  conj_transpose(n, d, at, a);
+  free(tau);
+  free(at);
+}
+
+/* cal_new_weight_T0 (src/mod_evolution.f90:359-408): the full-determinant
+ * weight of the current fields, from the trial strings at both ends. */
+void orc_cal_new_weight(orc_model *m, double *ri) {
+  int ne = m->ne, Ns = m->Ns, ng = m->ngroup;
+  zc *lmat = xcalloc((size_t)ne * Ns, sizeof(zc));
+  zc *tmat = xcalloc((size_t)ne * ne, sizeof(zc));
+  conj_transpose(Ns, ne, m->slQ, lmat); /* Q_string(:, :, nblock) */
+  zc lndet = 0;
+  for (int i = 0; i < ne; ++i) lndet = lndet + clog(m->slD[i]);
+  int flag = 0;
+  for (int p = m->Lt; p >= 1; --p) {
+    for (int f = 4; f >= 1; --f) orc_right_evolve(m, p, (double *)lmat, ne, f, 0);
+    flag = flag + 1;
+    if (flag != ng) continue;
+    flag = 0;
+    zlq(ne, Ns, lmat, tmat);
+    for (int i = 0; i < ne; ++i) lndet = lndet + clog(tmat[i + (size_t)ne * i]);
+  }
+  for (int i = 0; i < ne; ++i) lndet = lndet + clog(m->slD[i]); /* D_string(:, 1) */
+  gemm_nn(ne, ne, Ns, lmat, m->slQ, tmat);                      /* rmat = Q_string(:, :, 1) */
+  lndet = lndet + clog(zdet(ne, tmat));
+  if (m->tr_weight) lndet = lndet + conj(lndet);
+  m->ln_cw = ln_branch(0 + lndet * (double)m->ncopy);
+  ri[0] = creal(m->ln_cw);
+  ri[1] = cimag(m->ln_cw);
+  free(tmat);
+  free(lmat);
+}
+
+static float g_rands(orc_ran2 *s) {
+  float r;
+  orc_ran2_fill(s, 1, &r);
+  return r;
+}
+/* irands (src/mod_ranstate.f90:221-226): int(r * n) in REAL(SP) */
+static int g_irands(orc_ran2 *s, int n) { return (int)(g_rands(s) * (float)n); }
+static float g_ran_sysm(orc_ran2 *s) { return g_rands(s) * 2.0f - 1.0f; }
+
+/* generate_newfield_space (src/mod_update.f90:441-467) with
+ * K_vec = 2*PI*[0.5d0, 0.5d0] (PI in REAL(SP)) and phi = 0 */
+static void newfield_space(orc_model *m, double gud, const double *incell, orc_ran2 *s, double *bj) {
+  const double distance = 2 * gud / m->Ncell;
+  const double amplitude = distance * (double)g_rands(s);
+  const double K = (double)(2.0f * (float)SP_PI) * 0.5;
+  for (int c = 0; c < m->Ncell; ++c) {
+    int dp[2];
+    cell_dpos(m, c, dp); /* rpos = dpos for lat_vec = identity */
+    const double arg = (K * (double)dp[0] + K * (double)dp[1]) + 0.0;
+    for (int i = 0; i < 2; ++i) bj[2 * c + i] = incell[i] * amplitude * cos(arg);
+  }
+}
+
+/* updata_expKV_global(t0, t1): expKV of slices t0..t1 from the fields */
+static void expkv_slices(orc_model *m, int t0, int t1) {
+  for (int t = t0; t <= t1; ++t)
+    for (int f = 0; f < 4; ++f)
+      for (int b = 0; b < m->npla; ++b) {
+        double xv = m->x[m->bfi[f * m->npla + b] + (size_t)m->Nb * (t - 1)];
+        expkv(m, f, b, xv, 0, EKV(m, m->ekv, f, b, t));
+        expkv(m, f, b, xv, 1, EKV(m, m->ekvi, f, b, t));
+      }
+}
+
+/* generate_newfield_global (src/mod_update.f90:292-439) for the kinds the
+ * sweep loop calls (src/Main_PQMC.f90:88-94).  Returns -delt*dE. */
+static double newfield_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *s) {
+  const int Nb = m->Nb, Lt = m->Lt;
+  const double gud = gp->distance * m->char_length;
+  double *old = xcalloc((size_t)Nb * Lt, sizeof(double));
+  memcpy(old, m->x, sizeof(double) * Nb * Lt);
+  double de = 0.0;
+  if (kind == ORC_GU_KSPACE || kind == ORC_GU_KSPACE_TIME) {
+    double incell[2], *bj = xcalloc((size_t)Nb, sizeof(double));
+    for (int i = 0; i < 2; ++i) incell[i] = g_irands(s, 2) * 2.0 - 1.0;
+    newfield_space(m, gud, incell, s, bj);
+    int t0 = 1, t1 = Lt;
+    if (kind == ORC_GU_KSPACE_TIME) {
+      t0 = g_irands(s, Lt / 2) + 1;
+      t1 = t0 + (int)(1.0 * m->M / (m->dtau * m->dtau));
+      if (t1 < Lt) t1 = Lt;
+      /* the reference indexes past ntime when start + M/dtau^2 > ntime;
+       * the window is cut at ntime here */
+      if (t1 > Lt) t1 = Lt;
+    }
+    for (int t = t0; t <= t1; ++t)
+      for (int i = 0; i < Nb; ++i) m->x[i + (size_t)Nb * (t - 1)] = m->x[i + (size_t)Nb * (t - 1)] + bj[i];
+    for (int t = t0; t <= t1; ++t)
+      for (int i = 0; i < Nb; ++i) {
+        const double xn = m->x[i + (size_t)Nb * (t - 1)], xo = old[i + (size_t)Nb * (t - 1)];
+        de = de + 0.5 * m->D * (xn * xn - xo * xo);
+        de = de + (-m->biased * (xn - xo));
+      }
+    if (kind == ORC_GU_KSPACE_TIME) {
+      const double kin = 0.5 * (m->M / (m->dtau * m->dtau));
+      for (int i = 0; i < Nb; ++i) {
+#define XN(t) m->x[i + (size_t)Nb * ((t) - 1)]
+#define XO(t) old[i + (size_t)Nb * ((t) - 1)]
+        const int tb = modi(t0 - 1, Lt), ta = modi(t1 + 1, Lt);
+        de = de + kin * ((XN(t0) - XN(tb)) * (XN(t0) - XN(tb)) - (XO(t0) - XO(tb)) * (XO(t0) - XO(tb)));
+        de = de + kin * ((XN(t1) - XN(ta)) * (XN(t1) - XN(ta)) - (XO(t1) - XO(ta)) * (XO(t1) - XO(ta)));
+#undef XN
+#undef XO
+      }
+    }
+    free(bj);
+    expkv_slices(m, t0, t1);
+  } else if (kind == ORC_GU_ROTATE) {
+    const double ang = (double)g_ran_sysm(s) * 0.1 * (double)(float)SP_PI;
+    double r11 = cos(ang), r12 = -sin(ang), r21 = sin(ang), r22 = cos(ang);
+    if (g_ran_sysm(s) < 0.0f) {
+      r12 = -r12;
+      r22 = -r22;
+    }
+    for (int c = 0; c < m->Ncell; ++c)
+      for (int t = 0; t < Lt; ++t) {
+        double *x = &m->x[(size_t)Nb * t + 2 * c];
+        const double a = x[0], b = x[1];
+        x[0] = r11 * a + r12 * b;
+        x[1] = r21 * a + r22 * b;
+      }
+    expkv_slices(m, 1, Lt);
+  } else {
+    abort();
+  }
+  free(old);
+  return -de * m->dtau;
+}
+
+int orc_update_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *s) {
+  const size_t nx = (size_t)m->Nb * m->Lt, ne = (size_t)16 * m->npla * 4 * m->Lt;
+  double *x_old = xcalloc(nx, sizeof(double));
+  zc *e_old = xcalloc(ne, sizeof(zc)), *ei_old = xcalloc(ne, sizeof(zc));
+  memcpy(x_old, m->x, sizeof(double) * nx);
+  memcpy(e_old, m->ekv, sizeof(zc) * ne);
+  memcpy(ei_old, m->ekvi, sizeof(zc) * ne);
+  const zc w_old = m->ln_cw;
+  const double ph = newfield_global(m, kind, gp, s);
+  double ri[2];
+  orc_cal_new_weight(m, ri);
+  const double prob = cal_prob(m->ln_cw + ph - w_old);
+  const int accept = (double)g_rands(s) < prob;
+  if (accept) {
+    m->global_acc++;
+  } else {
+    memcpy(m->x, x_old, sizeof(double) * nx);
+    memcpy(m->ekv, e_old, sizeof(zc) * ne);
+    memcpy(m->ekvi, ei_old, sizeof(zc) * ne);
+    m->ln_cw = w_old;
+    m->global_rej++;
+  }
+  free(ei_old);
+  free(e_old);
+  free(x_old);
+  return accept;
+}
+
+void orc_propose_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *s, double *x_out, double *ph_out) {
+  const size_t nx = (size_t)m->Nb * m->Lt, ne = (size_t)16 * m->npla * 4 * m->Lt;
+  double *x_old = xcalloc(nx, sizeof(double));
+  zc *e_old = xcalloc(ne, sizeof(zc)), *ei_old = xcalloc(ne, sizeof(zc));
+  memcpy(x_old, m->x, sizeof(double) * nx);
+  memcpy(e_old, m->ekv, sizeof(zc) * ne);
+  memcpy(ei_old, m->ekvi, sizeof(zc) * ne);
+  *ph_out = newfield_global(m, kind, gp, s);
+  memcpy(x_out, m->x, sizeof(double) * nx);
+  memcpy(m->x, x_old, sizeof(double) * nx);
+  memcpy(m->ekv, e_old, sizeof(zc) * ne);
+  memcpy(m->ekvi, ei_old, sizeof(zc) * ne);
+  free(ei_old);
+  free(e_old);
+  free(x_old);
+}
+
+void orc_get_global_counters(const orc_model *m, long *acc, long *rej) {
+  *acc = m->global_acc;
+  *rej = m->global_rej;
 }
 
 /* ------------------------------------------------------------------ */

@@ -97,6 +97,20 @@ typedef struct { int32_t i, j, k, m, n; } orc_ran2;
 void orc_ran2_seed(orc_ran2 *s, int32_t seed);
 void orc_ran2_fill(orc_ran2 *s, int64_t n, float *out);
 
+/* global moves: cal_new_weight_T0 (src/mod_evolution.f90:359-408) on the
+ * current fields (sets ln_cw), and update_global (src/mod_update.f90:251-290)
+ * for the kinds the sweep loop calls (src/Main_PQMC.f90:88-94), drawing from
+ * rng in the reference's order; returns 1 if the move was accepted. */
+enum { ORC_GU_KSPACE = 0, ORC_GU_ROTATE = 1, ORC_GU_KSPACE_TIME = 2 };
+typedef struct {
+  double distance; /* global_update_distance before the char_length scaling */
+} orc_global;
+void orc_cal_new_weight(orc_model *m, double *re_im);
+int orc_update_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *rng);
+void orc_get_global_counters(const orc_model *m, long *accepted, long *rejected);
+/* the proposal of update_global alone (fields and -delt*dE), model left unchanged */
+void orc_propose_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *rng, double *x_out, double *ph_out);
+
 /* dense kernels (exposed for tests) */
 void orc_zqdr(int m, int n, double *a_c128, double *r_c128, double *d_c128);
 void orc_zldq(int m, int n, double *a_c128, double *l_c128, double *d_c128);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "pqmc_get_qstring": (_I, [_P, _I, _P]),
     "pqmc_get_dstring": (_I, [_P, _P]),
     "pqmc_get_lncw": (_I, [_P, _P]),
+    "pqmc_global_weight": (_I, [_P, _P, _P]),
     "pqmc_get_accepts": (_I, [_P, _P, C.c_int64]),
     "pqmc_get_counters": (_I, [_P, _P, _P]),
     "pqmc_synchronize": (_I, [_P]),

@@ -114,6 +114,11 @@ struct pqmc_handle {
   unsigned long long *d_stats = nullptr, *d_err = nullptr;
   zd *d_meas = nullptr, *d_kphase = nullptr;
   unsigned long long *d_prof = nullptr;  // family-pass phase timers (debug)
+  // global-move weight (allocated on first use): proposed fields and their exponentials
+  double *d_gx = nullptr;
+  ekv_t *d_gekv = nullptr;
+  zd *d_glog = nullptr;  // [nblock + 4] log terms of the weight
+  zd *d_gw = nullptr;    // the weight
   double err_fast_max = 1e-6;
   // timing
   bool timing[3] = {false, false, false};
@@ -299,7 +304,8 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->d_Q, h->d_D, h->d_R, h->d_dlog, h->d_rinv, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
-                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof};
+                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
+                  h->d_gx, h->d_gekv, h->d_glog, h->d_gw};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -420,6 +426,9 @@ static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout) {
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRm, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
   HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_dlog, h->nblock, h->md.tr_weight, h->md.ncopy,
                    h->d_lncw, h->st));
+  // the accepts' log(real(R_elec)) terms since the last stabilization are
+  // overwritten with it (src/mod_update.f90:142)
+  HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
   return PQMC_OK;
 }
 
@@ -427,13 +436,13 @@ static zd *QS(pqmc_handle *h, int block) { return h->d_Q + (size_t)h->Ns * h->ne
 static zd *DS(pqmc_handle *h, int block) { return h->d_D + (size_t)h->ne * (block - 1); }
 static zd *RS(pqmc_handle *h, int block) { return h->d_R + (size_t)h->ne * h->ne * (block - 1); }
 
-static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backward) {
+static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backward, const ekv_t *ekv = nullptr) {
   prop_args a;
   a.md = h->md;
   a.X = X;
   a.ncols = h->ne;
   a.sites = h->d_sites;
-  a.ekv = h->d_ekv;
+  a.ekv = ekv ? ekv : h->d_ekv;
   a.t_first = t_first;
   a.nslices = nslices;
   a.backward = backward;
@@ -787,7 +796,58 @@ extern "C" int pqmc_get_dstring(pqmc_handle *h, double *d) {
 extern "C" int pqmc_get_lncw(pqmc_handle *h, double *ri) {
   if (!h || !ri) return fail(PQMC_EINVAL, "null argument");
   zd v;
+  double acc = 0.0;
   HIPCHK(hipMemcpyAsync(&v, h->d_lncw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(&acc, h->d_lncw_acc, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  ri[0] = v.x + acc;
+  ri[1] = v.y;
+  return PQMC_OK;
+}
+
+// cal_new_weight_T0 (src/mod_evolution.f90:359-408) for the fields x: the
+// left trial string propagated down through all slices with an LQ every
+// ngroup slices (the QR of its conjugate transpose, whose real diagonal is
+// the same), then log det of its product with the right trial string, plus
+// the trial's log D at both ends.  The chain's own state is not touched.
+extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
+  if (!h || !x || !ri) return fail(PQMC_EINVAL, "null argument");
+  if (!h->model_set || !h->trial_set) return fail(PQMC_EINVAL, "model and trial must be set");
+  const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, Lt = h->Lt;
+  const size_t nx = (size_t)h->Nb * Lt;
+  if (!h->d_gx) {
+    HIPCHK(dalloc(&h->d_gx, nx));
+    HIPCHK(dalloc(&h->d_gekv, (size_t)Lt * 4 * h->npla));
+    HIPCHK(dalloc(&h->d_glog, (size_t)h->nblock + 4));
+    HIPCHK(dalloc(&h->d_gw, 1));
+  }
+  HIPCHK(hipMemcpyAsync(h->d_gx, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->st));
+  HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_gx, h->d_bfi, h->d_cross, h->d_ph, h->d_gekv, 1, Lt, h->st));
+  const size_t qb = sizeof(zd) * Ns * ne;
+  zd *W = h->d_W, *Qn = h->d_scr;  // string (as lmat^H) and the QR's explicit Q
+  HIPCHK(hipMemcpyAsync(W, h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(pqmc_dlog(h->d_slD, ne, h->d_glog, h->st));      // D_string(:, nblock)
+  HIPCHK(pqmc_dlog(h->d_slD, ne, h->d_glog + 1, h->st));  // D_string(:, 1)
+  int nlog = 2, rc = PQMC_OK;
+  for (int p = Lt; p >= 1;) {
+    const int n = std::min(ng, p);
+    rc = propagate(h, W, p, n, 1, h->d_gekv);
+    if (rc) return rc;
+    p -= n;
+    if (n < ng) break;  // a partial last group has no LQ
+    rc = run_qr(h, h->qa, W, Ns, Ns, ne, Qn, nullptr, nullptr, 0);
+    if (rc) return rc;
+    HIPCHK(pqmc_dlog(h->qa.beta, ne, h->d_glog + nlog++, h->st));
+    std::swap(W, Qn);
+  }
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  HIPCHK(pqmc_zgemm(1, 0, ne, ne, Ns, one, W, Ns, h->d_slQ, Ns, zero, h->d_M, ne, h->st));
+  rc = run_qr(h, h->qm, h->d_M, ne, ne, ne, nullptr, nullptr, nullptr, 0);
+  if (rc) return rc;
+  HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_glog, nlog, h->md.tr_weight, h->md.ncopy, h->d_gw,
+                   h->st));
+  zd v;
+  HIPCHK(hipMemcpyAsync(&v, h->d_gw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   ri[0] = v.x;
   ri[1] = v.y;

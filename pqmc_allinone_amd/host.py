@@ -23,6 +23,7 @@ reference would ``stop``.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -69,6 +70,77 @@ class Ran2:
     def ran_sysm(self) -> float:
         r = self.draws(1)[0]
         return float(np.float32(r * np.float32(2) - np.float32(1)))
+
+
+# REAL(SP) PI of src/mod_nrtype.f90:42
+SP_PI = np.float32(np.pi)
+# update_global kinds in the order the sweep loop calls them (src/Main_PQMC.f90:88-94)
+GLOBAL_KINDS = ("kspace", "rotate", "kspace_time")
+
+
+def _irands(rng: "Ran2", n: int) -> int:
+    """irands (src/mod_ranstate.f90:221-226): int(r * n) in REAL(SP)."""
+    return int(np.float32(rng.draws(1)[0]) * np.float32(n))
+
+
+def _seqsum(start: float, terms: np.ndarray) -> float:
+    """start + t1 + t2 + ... added left to right, as the reference's loops do."""
+    return float(np.add.accumulate(np.concatenate(([start], np.ravel(terms))))[-1])
+
+
+def propose_global(model: Model, x: np.ndarray, kind: str, rng: "Ran2", distance: float = 1.0):
+    """generate_newfield_global (src/mod_update.f90:292-439) for ``kind``.
+
+    Returns the proposed fields (Lt x Nb, row t = boson_field(:, t)) and
+    -delt*dE, the bosonic part of the weight change.  Draws come from ``rng``
+    in the reference's order.  Arithmetic keeps the reference's REAL(SP)
+    steps (PI, ran_sysm, irands) and its summation order.  For kspace_time
+    the window is cut at ntime, where the reference would index past it
+    (start_time + M/dtau^2 > ntime).
+    """
+    p = model.p
+    Lt, Nb = p.Lt, model.Nb
+    xo = np.ascontiguousarray(x, np.float64).reshape(Lt, Nb)
+    xn = xo.copy()
+    D, M, dt = model.D, model.M, p.dtau
+    if kind in ("kspace", "kspace_time"):
+        incell = [_irands(rng, 2) * 2.0 - 1.0 for _ in range(2)]
+        # generate_newfield_space (:441-467) with K_vec = 2*PI*[0.5d0, 0.5d0], phi = 0
+        gud = distance * model.char_length
+        dist = 2 * gud / model.Ncell
+        amplitude = dist * float(rng.draws(1)[0])
+        K = float(np.float32(2) * SP_PI) * 0.5
+        bj = np.empty(Nb)
+        for c in range(model.Ncell):
+            cx, cy = c % p.L1, c // p.L1  # rpos = dpos (0-based) for lat_vec = identity
+            cs = math.cos((K * cx + K * cy) + 0.0)
+            for i in range(2):
+                bj[2 * c + i] = incell[i] * amplitude * cs
+        t0, t1 = 1, Lt
+        if kind == "kspace_time":
+            t0 = _irands(rng, Lt // 2) + 1
+            t1 = min(max(t0 + int(1.0 * M / (dt * dt)), Lt), Lt)
+        xn[t0 - 1:t1] = xo[t0 - 1:t1] + bj
+        a, b = xn[t0 - 1:t1], xo[t0 - 1:t1]
+        terms = np.stack([0.5 * D * (a * a - b * b), -p.biased * (a - b)], axis=-1)
+        de = _seqsum(0.0, terms)
+        if kind == "kspace_time":
+            kin = 0.5 * (M / (dt * dt))
+            tb, ta = (t0 - 2) % Lt, t1 % Lt  # modi(t0 - 1, ntime), modi(t1 + 1, ntime), 0-based rows
+            e0 = kin * ((xn[t0 - 1] - xn[tb]) ** 2 - (xo[t0 - 1] - xo[tb]) ** 2)
+            e1 = kin * ((xn[t1 - 1] - xn[ta]) ** 2 - (xo[t1 - 1] - xo[ta]) ** 2)
+            de = _seqsum(de, np.stack([e0, e1], axis=-1))
+        return xn, -de * dt
+    if kind == "rotate":
+        ang = float(np.float32(rng.ran_sysm())) * 0.1 * float(SP_PI)
+        r11, r12, r21, r22 = math.cos(ang), -math.sin(ang), math.sin(ang), math.cos(ang)
+        if rng.ran_sysm() < 0.0:
+            r12, r22 = -r12, -r22
+        a, b = xo[:, 0::2], xo[:, 1::2]
+        xn[:, 0::2] = r11 * a + r12 * b
+        xn[:, 1::2] = r21 * a + r22 * b
+        return xn, 0.0
+    raise ValueError(f"unknown global update kind {kind!r}")
 
 
 class Chain:
@@ -190,6 +262,15 @@ class Chain:
         _lib.check(self._L.pqmc_get_lncw(self.h, _lib.ptr(v)))
         return complex(v[0], v[1])
 
+    def global_weight(self, x: np.ndarray) -> complex:
+        """cal_new_weight_T0 (src/mod_evolution.f90:359-408) for the fields x."""
+        x = np.ascontiguousarray(x, np.float64)
+        if x.size != self.Nb * self.model.p.Lt:
+            raise ValueError("fields must hold Nb*Lt values")
+        v = np.zeros(2)
+        _lib.check(self._L.pqmc_global_weight(self.h, _lib.ptr(x), _lib.ptr(v)))
+        return complex(v[0], v[1])
+
     def q_string(self, block: int) -> np.ndarray:
         q = np.empty((self.Ns, self.model.p.ne), np.complex128, order="F")
         _lib.check(self._L.pqmc_get_qstring(self.h, int(block), _lib.ptr(q)))
@@ -279,3 +360,41 @@ def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
         cnt = np.fromfile(f, np.int64, 2)
     return {"accepts": acc, "g": g, "fields": fields, "meas_sum": meas, "n_meas": n_meas,
             "positive_accept": int(cnt[0]), "negative_accept": int(cnt[1])}
+
+
+class GlobalMoves:
+    """The global phonon moves of the sweep loop (src/Main_PQMC.f90:87-95).
+
+    ``update(kind)`` is update_global (src/mod_update.f90:251-290): propose,
+    weigh the new fields on the GPU (cal_new_weight_T0), and accept with
+    cal_prob when rands() < prob.  ``run()`` is the block the sweep loop runs
+    before a backward sweep: kspace, rotate and kspace_time, then init_g(1)
+    if any of them was accepted.
+    """
+
+    def __init__(self, chain: Chain, rng: Ran2, distance: float = 1.0):
+        self.chain, self.rng, self.distance = chain, rng, distance
+        self.accepted = 0
+        self.rejected = 0
+
+    def update(self, kind: str, x: np.ndarray, lncw: complex):
+        x_new, ph = propose_global(self.chain.model, x, kind, self.rng, self.distance)
+        w = self.chain.global_weight(x_new)
+        prob = min(1.0, math.exp(min(10.0, (w + ph - lncw).real)))
+        if float(self.rng.draws(1)[0]) < prob:
+            self.accepted += 1
+            return True, x_new, w
+        self.rejected += 1
+        return False, x, lncw
+
+    def run(self):
+        x = self.chain.get_fields()
+        lncw = self.chain.ln_cw
+        flags = []
+        for kind in GLOBAL_KINDS:
+            acc, x, lncw = self.update(kind, x, lncw)
+            flags.append(acc)
+        if any(flags):
+            self.chain.set_fields(x)
+            self.chain.init_g(1)
+        return flags

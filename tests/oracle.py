@@ -66,6 +66,12 @@ def lib():
         L.orc_zldq.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_zinverse.argtypes = [C.c_int, C.c_void_p]
         L.orc_zdet.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_cal_new_weight.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_update_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(GlobalParams), C.POINTER(Ran2State)]
+        L.orc_update_global.restype = C.c_int
+        L.orc_get_global_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_propose_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(GlobalParams), C.POINTER(Ran2State),
+                                         C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -80,6 +86,14 @@ def ran2(seed: int, n: int) -> np.ndarray:
     out = np.empty(int(n), np.float32)
     lib().orc_ran2_fill(C.byref(s), int(n), P(out))
     return out
+
+
+class GlobalParams(C.Structure):
+    _fields_ = [("distance", C.c_double)]
+
+
+# kinds of update_global in the order the sweep loop calls them (src/Main_PQMC.f90:88-94)
+GU_KSPACE, GU_ROTATE, GU_KSPACE_TIME = 0, 1, 2
 
 
 class Ran2Stream:
@@ -138,6 +152,9 @@ class Oracle:
     def init(self, start_time=1):
         lib().orc_init_expkv(self.m)
         lib().orc_init_g(self.m, int(start_time))
+
+    def init_expkv(self):
+        lib().orc_init_expkv(self.m)
 
     def expkv(self, pf_id, pla, x, inv):
         e = np.zeros((4, 4), np.complex128, order="F")
@@ -208,6 +225,27 @@ class Oracle:
         lib().orc_sweep(self.m, int(bool(forward)), P(u), int(ntc), P(acc), P(out) if measure else None,
                         C.addressof(nm))
         return acc, out
+
+    def cal_new_weight(self):
+        out = np.zeros(2)
+        lib().orc_cal_new_weight(self.m, P(out))
+        return complex(out[0], out[1])
+
+    def update_global(self, kind, rng, distance=1.0):
+        gp = GlobalParams(float(distance))
+        return bool(lib().orc_update_global(self.m, int(kind), C.byref(gp), C.byref(rng.s)))
+
+    def propose_global(self, kind, rng, distance=1.0):
+        gp = GlobalParams(float(distance))
+        x = np.zeros((self.Lt, self.Nb))
+        ph = C.c_double(0)
+        lib().orc_propose_global(self.m, int(kind), C.byref(gp), C.byref(rng.s), P(x), C.addressof(ph))
+        return x, ph.value
+
+    def global_counters(self):
+        a, r = C.c_long(0), C.c_long(0)
+        lib().orc_get_global_counters(self.m, C.addressof(a), C.addressof(r))
+        return a.value, r.value
 
     def counters(self):
         pa, na, efm = C.c_long(0), C.c_long(0), C.c_double(0)
