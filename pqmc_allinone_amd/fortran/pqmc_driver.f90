@@ -1,6 +1,7 @@
 ! Sweep driver over the drop-in subroutines: the loop body of
-! src/Main_PQMC.f90:81-156 with the global update off, called one reference
-! subroutine at a time (update_phonon_field, right_/left_evolve on g_h,
+! src/Main_PQMC.f90:81-156, called one reference subroutine at a time
+! (update_global before a backward sweep when mod(loop + 1,
+! global_update_loop) == 0, update_phonon_field, right_/left_evolve on g_h,
 ! get_g_scratch every ngroup slices, begin_measure at ntime/2).
 !
 ! usage: pqmc_driver <input file> <output file>
@@ -8,11 +9,13 @@
 ! seed; int32 sites[4][npla][4], bfi[4][npla], cross[4][npla]; complex128
 ! bcph[4][npla][16], kcoe[4], vcoe[4], k12[4][4], v12[4][4], bc[2]; float64
 ! dtau M D biased jump_distance hop lam; int32 ncopy tr_weight; float64
-! fields(Nb, Lt); complex128 slater(Ns, ne).
+! fields(Nb, Lt); complex128 slater(Ns, ne); int32 global_update_loop (0 =
+! global update off); float64 global_update_distance (times char_length).
 ! Output: int8 accept flags of every update call in call order
 ! (nsweeps * Lt * 4 * npla); complex128 G(Ns, Ns) at the end; float64
 ! fields(Nb, Lt); complex128 summed measurements (total_width); int32 number
-! of measurements; int64 positive and negative accepts.
+! of measurements; int64 positive and negative accepts; int64 global
+! accepts and rejects.
 program pqmc_driver
   use iso_c_binding
   use pqmc_hip
@@ -20,7 +23,8 @@ program pqmc_driver
   implicit none
   character(len=1024) :: fin, fout
   integer :: u_in, u_out
-  integer(c_int32_t) :: hdr(8), ncopy_tr(2)
+  integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop
+  real(c_double) :: glob_dist
   integer :: L1, L2, Lt, ng, ne, nbb, nsweeps, seed, npla_, ns_, nb_
   integer(c_int32_t), allocatable, target :: sites(:), bfi(:), cross(:)
   complex(c_double_complex), allocatable, target :: bcph(:), kcoe(:), vcoe(:), k12(:), v12(:), bc(:)
@@ -46,7 +50,7 @@ program pqmc_driver
   npla_ = ns_ / 4
   allocate (sites(4 * npla_ * 4), bfi(4 * npla_), cross(4 * npla_), bcph(4 * npla_ * 16), kcoe(4), vcoe(4), &
             k12(16), v12(16), bc(2), fields(nb_ * Lt), slater(ns_ * ne))
-  read (u_in) sites, bfi, cross, bcph, kcoe, vcoe, k12, v12, bc, reals, ncopy_tr, fields, slater
+  read (u_in) sites, bfi, cross, bcph, kcoe, vcoe, k12, v12, bc, reals, ncopy_tr, fields, slater, glob_loop, glob_dist
   close (u_in)
 
   cfg = pqmc_config(L1, L2, Lt, ng, ne, nbb)
@@ -56,7 +60,7 @@ program pqmc_driver
   model%dtau = reals(1); model%M = reals(2); model%D = reals(3); model%biased = reals(4)
   model%jump_distance = reals(5); model%hop = reals(6); model%lam = reals(7)
   model%ncopy = ncopy_tr(1); model%tr_weight = ncopy_tr(2)
-  call pqmc_open(cfg, model, fields, slater, seed)
+  call pqmc_open(cfg, model, fields, slater, seed, glob_dist)
   do k = 1, 4
     ppf_list(k)%id = k
   end do
@@ -67,6 +71,15 @@ program pqmc_driver
   forward = .true.
   do loop = 1, nsweeps
     forward = .not. forward  ! the first loop is going backwards
+    if ((.not. forward) .and. glob_loop > 0) then
+      if (mod(loop + 1, int(glob_loop)) == 0) then
+        updated = .false.
+        call update_global("kspace")
+        call update_global("rotate")
+        call update_global("kspace_time")
+        if (updated) call init_g(1)
+      end if
+    end if
     do time_count = 1, ntime
       if (.not. forward) then
         time = ntime - time_count + 1
@@ -100,7 +113,7 @@ program pqmc_driver
   call pqmc_check(pqmc_get_fields(chain, fields), 'pqmc_get_fields')
   call pqmc_check(pqmc_get_counters(chain, cnt, efm), 'pqmc_get_counters')
   open (newunit=u_out, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write (u_out) acc_all, g_h, fields, meas_sum, int(n_meas, c_int32_t), cnt(1), cnt(2)
+  write (u_out) acc_all, g_h, fields, meas_sum, int(n_meas, c_int32_t), cnt(1), cnt(2), global_accept, global_reject
   close (u_out)
   call pqmc_close()
 end program pqmc_driver

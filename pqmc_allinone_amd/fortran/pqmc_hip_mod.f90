@@ -150,6 +150,13 @@ module pqmc_hip
       real(c_double), intent(out) :: re_im(2)
     end function pqmc_get_lncw
 
+    integer(c_int) function pqmc_global_weight(h, x, re_im) bind(C, name='pqmc_global_weight')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: re_im(2)
+    end function pqmc_global_weight
+
     integer(c_int) function pqmc_get_accepts(h, acc, n) bind(C, name='pqmc_get_accepts')
       import :: c_int, c_ptr, c_int8_t, c_int64_t
       type(c_ptr), value :: h

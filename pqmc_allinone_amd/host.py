@@ -328,8 +328,10 @@ FORTRAN_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortr
 
 
 def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.ndarray, nsweeps: int, seed: int,
-                       block_bonds: int = 0):
-    """Input file of fortran/pqmc_driver (layout in its header comment)."""
+                       block_bonds: int = 0, global_update_loop: int = 0, global_distance: float = 1.0):
+    """Input file of fortran/pqmc_driver (layout in its header comment).
+    global_distance is the input's global_update_distance, before its scaling
+    by char_length (src/input.f90:184)."""
     p = model.p
     with open(path, "wb") as f:
         np.array([p.L1, p.L2, p.Lt, p.ngroup, p.ne, block_bonds, nsweeps, seed], np.int32).tofile(f)
@@ -346,6 +348,8 @@ def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.n
         np.array([p.ncopy, int(bool(p.tr_weight))], np.int32).tofile(f)
         np.ascontiguousarray(fields, np.float64).tofile(f)
         np.asfortranarray(slater, np.complex128).ravel(order="F").tofile(f)
+        np.array([global_update_loop], np.int32).tofile(f)
+        np.array([global_distance * model.char_length], np.float64).tofile(f)
 
 
 def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
@@ -357,9 +361,10 @@ def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
         fields = np.fromfile(f, np.float64, model.Nb * p.Lt).reshape((p.Lt, model.Nb))
         meas = np.fromfile(f, np.complex128, model.total_width)
         n_meas = int(np.fromfile(f, np.int32, 1)[0])
-        cnt = np.fromfile(f, np.int64, 2)
+        cnt = np.fromfile(f, np.int64, 4)
     return {"accepts": acc, "g": g, "fields": fields, "meas_sum": meas, "n_meas": n_meas,
-            "positive_accept": int(cnt[0]), "negative_accept": int(cnt[1])}
+            "positive_accept": int(cnt[0]), "negative_accept": int(cnt[1]),
+            "global_accept": int(cnt[2]), "global_reject": int(cnt[3])}
 
 
 class GlobalMoves:
