@@ -121,6 +121,11 @@ int pqmc_get_lncw(pqmc_handle *h, double *re_im);
 int pqmc_get_accepts(pqmc_handle *h, uint8_t *acc, int64_t n); /* last sweep, call order */
 /* positive_accept, negative_accept, sync timeouts; err_fast_max. */
 int pqmc_get_counters(pqmc_handle *h, int64_t *out3, double *err_fast_max);
+/* Sets positive_accept, negative_accept (in2) and err_fast_max, for a run
+ * resumed from a checkpoint.  No reference counterpart: the reference keeps
+ * these as module globals (src/mod_update.f90, src/Main_PQMC.f90:222-224)
+ * and has no resume. */
+int pqmc_set_counters(pqmc_handle *h, const int64_t *in2, double err_fast_max);
 int pqmc_synchronize(pqmc_handle *h);
 
 /* Global moves: the full-determinant weight of the field configuration x

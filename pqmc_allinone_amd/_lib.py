@@ -65,6 +65,7 @@ SIGNATURES = {
     "pqmc_global_weight": (_I, [_P, _P, _P]),
     "pqmc_get_accepts": (_I, [_P, _P, C.c_int64]),
     "pqmc_get_counters": (_I, [_P, _P, _P]),
+    "pqmc_set_counters": (_I, [_P, _P, C.c_double]),
     "pqmc_synchronize": (_I, [_P]),
     "pqmc_ran2_seed": (None, [C.POINTER(Ran2), C.c_int32]),
     "pqmc_ran2_fill": (None, [C.POINTER(Ran2), C.c_int64, _P]),

@@ -874,6 +874,18 @@ extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
   return PQMC_OK;
 }
 
+// Restores positive_accept, negative_accept and err_fast_max (resume from a
+// checkpoint); the sync-timeout counter is left as it is.
+extern "C" int pqmc_set_counters(pqmc_handle *h, const int64_t *in2, double err_fast_max) {
+  if (!h || !in2) return fail(PQMC_EINVAL, "null argument");
+  if (in2[0] < 0 || in2[1] < 0 || !(err_fast_max >= 0.0)) return fail(PQMC_EINVAL, "negative counter");
+  unsigned long long s[2] = {(unsigned long long)in2[0], (unsigned long long)in2[1]};
+  HIPCHK(hipMemcpyAsync(h->d_stats, s, sizeof(s), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->err_fast_max = err_fast_max;
+  return PQMC_OK;
+}
+
 // Phase timers of the family pass (debug): enable != 0 allocates and
 // zeroes them; out (8*P values, may be NULL) receives microseconds.  Rows:
 // leader wave 0, the H helpers, leader wave 1, leader wave 2.

@@ -2,15 +2,22 @@
 ! src/Main_PQMC.f90:81-156, called one reference subroutine at a time
 ! (update_global before a backward sweep when mod(loop + 1,
 ! global_update_loop) == 0, update_phonon_field, right_/left_evolve on g_h,
-! get_g_scratch every ngroup slices, begin_measure at ntime/2).
+! get_g_scratch every ngroup slices, begin_measure around ntime/2 after the
+! warm-up, src/Main_PQMC.f90:143-148).
 !
-! usage: pqmc_driver <input file> <output file>
+! usage: pqmc_driver <input file> <output file> [<output folder>/]
+! The output folder (with temp/ and data/ in it) receives the measurement
+! bins and files when nbins > 0, and checkpoint.bin when a checkpoint is
+! saved or resumed from.
 ! Input (stream, little-endian): int32 L1 L2 Lt ngroup ne block_bonds nsweeps
 ! seed; int32 sites[4][npla][4], bfi[4][npla], cross[4][npla]; complex128
 ! bcph[4][npla][16], kcoe[4], vcoe[4], k12[4][4], v12[4][4], bc[2]; float64
 ! dtau M D biased jump_distance hop lam; int32 ncopy tr_weight; float64
 ! fields(Nb, Lt); complex128 slater(Ns, ne); int32 global_update_loop (0 =
-! global update off); float64 global_update_distance (times char_length).
+! global update off); float64 global_update_distance (times char_length);
+! int32 nbins (0 = no bins or files), nmeas_per_bin, meas_number, warmup,
+! meas_interval, checkpoint loop (save after it, 0 = none), resume (1 =
+! start from checkpoint.bin).
 ! Output: int8 accept flags of every update call in call order
 ! (nsweeps * Lt * 4 * npla); complex128 G(Ns, Ns) at the end; float64
 ! fields(Nb, Lt); complex128 summed measurements (total_width); int32 number
@@ -23,7 +30,10 @@ program pqmc_driver
   implicit none
   character(len=1024) :: fin, fout
   integer :: u_in, u_out
-  integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop
+  integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop, mc(7)
+  character(len=1024) :: outdir
+  integer :: nbins, nmeas, mnum, warmup, meas_interval, ckpt_loop, loop0
+  logical :: resume
   real(c_double) :: glob_dist
   integer :: L1, L2, Lt, ng, ne, nbb, nsweeps, seed, npla_, ns_, nb_
   integer(c_int32_t), allocatable, target :: sites(:), bfi(:), cross(:)
@@ -50,7 +60,13 @@ program pqmc_driver
   npla_ = ns_ / 4
   allocate (sites(4 * npla_ * 4), bfi(4 * npla_), cross(4 * npla_), bcph(4 * npla_ * 16), kcoe(4), vcoe(4), &
             k12(16), v12(16), bc(2), fields(nb_ * Lt), slater(ns_ * ne))
-  read (u_in) sites, bfi, cross, bcph, kcoe, vcoe, k12, v12, bc, reals, ncopy_tr, fields, slater, glob_loop, glob_dist
+  read (u_in) sites, bfi, cross, bcph, kcoe, vcoe, k12, v12, bc, reals, ncopy_tr, fields, slater, glob_loop, glob_dist, mc
+  nbins = mc(1); nmeas = mc(2); mnum = mc(3); warmup = mc(4); meas_interval = mc(5); ckpt_loop = mc(6)
+  resume = mc(7) /= 0
+  outdir = ''
+  if (command_argument_count() >= 3) call get_command_argument(3, outdir)
+  if ((nbins > 0 .or. ckpt_loop > 0 .or. resume) .and. len_trim(outdir) == 0) &
+    error stop 'pqmc_driver: bins and checkpoints need an output folder'
   close (u_in)
 
   cfg = pqmc_config(L1, L2, Lt, ng, ne, nbb)
@@ -64,12 +80,18 @@ program pqmc_driver
   do k = 1, 4
     ppf_list(k)%id = k
   end do
-  call init_g(1)
-
+  if (nbins > 0) call meas_init(nbins, nmeas, mnum, trim(outdir), 0, 1, 1)
   allocate (acc_all(nsweeps * Lt * 4 * n_pla))
-  pos = 0
+  acc_all = 0
+  loop0 = 0
   forward = .true.
-  do loop = 1, nsweeps
+  if (resume) then
+    call checkpoint_load(trim(outdir)//'checkpoint.bin', loop0, forward)
+  else
+    call init_g(1)
+  end if
+  pos = loop0 * Lt * 4 * n_pla
+  do loop = loop0 + 1, nsweeps
     forward = .not. forward  ! the first loop is going backwards
     if ((.not. forward) .and. glob_loop > 0) then
       if (mod(loop + 1, int(glob_loop)) == 0) then
@@ -105,9 +127,13 @@ program pqmc_driver
           call get_g_scratch(time, forward, loop)
         end if
       end if
-      if (time == ntime / 2) call begin_measure(forward, time)
+      if (loop > warmup .and. mod(loop - warmup, meas_interval) == 0) then
+        if (time >= ntime / 2 - meas_number .and. time <= ntime / 2 + meas_number) call begin_measure(forward, time)
+      end if
     end do
+    if (loop == ckpt_loop) call checkpoint_save(trim(outdir)//'checkpoint.bin', loop, forward)
   end do
+  if (nbins > 0) call data_analyse()
 
   call sync_g_h()
   call pqmc_check(pqmc_get_fields(chain, fields), 'pqmc_get_fields')

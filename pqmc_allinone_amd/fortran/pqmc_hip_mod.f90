@@ -171,6 +171,13 @@ module pqmc_hip
       real(c_double), intent(out) :: err_fast_max
     end function pqmc_get_counters
 
+    integer(c_int) function pqmc_set_counters(h, in2, err_fast_max) bind(C, name='pqmc_set_counters')
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), intent(in) :: in2(2)
+      real(c_double), value :: err_fast_max
+    end function pqmc_set_counters
+
     integer(c_int) function pqmc_synchronize(h) bind(C, name='pqmc_synchronize')
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -11,7 +11,8 @@ update_phonon_field(ppf,t)   ``Chain.update_phonon_field(pf_id, time, u)``
 left_evolve(t,g_h,..,.true.) ``Chain.left_evolve(time, pf_id)``
 right_evolve(t,g_h,..)       ``Chain.right_evolve(time, pf_id)``
 get_g_scratch(t,fwd,loop)    ``Chain.get_g_scratch(time, forward)``
-Meas_sys%begin_measure       ``Measurements.begin_measure(chain, fwd, t)``
+Meas_sys%begin_measure       ``measio.MeasurementSystem.begin_measure``
+update_global(kind)          ``GlobalMoves.update(kind, x, ln_cw)``
 loop body (one sweep)        ``Chain.sweep(forward, draws, measure)``
 ran2_s / rands / ran_sysm    ``Ran2`` (src/mod_ranstate.f90:171-236)
 ===========================  ============================================
@@ -294,6 +295,11 @@ class Chain:
         return {"positive_accept": int(o[0]), "negative_accept": int(o[1]), "sync_timeouts": int(o[2]),
                 "err_fast_max": efm.value}
 
+    def set_counters(self, positive_accept: int, negative_accept: int, err_fast_max: float):
+        """Restore the accept counters and err_fast_max (resume from a checkpoint)."""
+        v = np.array([positive_accept, negative_accept], np.int64)
+        _lib.check(self._L.pqmc_set_counters(self.h, _lib.ptr(v), float(err_fast_max)))
+
     def synchronize(self):
         _lib.check(self._L.pqmc_synchronize(self.h))
 
@@ -304,31 +310,15 @@ class Chain:
         return ms.value, n.value
 
 
-class Measurements:
-    """Binning of begin_measure / compute_one_bin_mean (src/mod_meas.f90:729-763, 847-858)."""
-
-    def __init__(self, chain: Chain, nmeas_per_bin: int):
-        self.chain = chain
-        self.nmeas = int(nmeas_per_bin)
-        self.cur = []
-        self.bin_means = []
-
-    def begin_measure(self, forward: bool, time: int):
-        data = self.chain.take_measurement(time)
-        self.cur.append(data)
-        if len(self.cur) == self.nmeas:
-            self.bin_means.append(np.mean(np.real(np.array(self.cur)), axis=0))
-            self.cur = []
-        return data
-
-
 # ---- Fortran host driver (fortran/pqmc_driver) file formats ---------------
 
 FORTRAN_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "pqmc_driver")
 
 
 def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.ndarray, nsweeps: int, seed: int,
-                       block_bonds: int = 0, global_update_loop: int = 0, global_distance: float = 1.0):
+                       block_bonds: int = 0, global_update_loop: int = 0, global_distance: float = 1.0,
+                       nbins: int = 0, nmeas_per_bin: int = 0, meas_number: int = 0, warmup: int = 0,
+                       meas_interval: int = 1, checkpoint_loop: int = 0, resume: bool = False):
     """Input file of fortran/pqmc_driver (layout in its header comment).
     global_distance is the input's global_update_distance, before its scaling
     by char_length (src/input.f90:184)."""
@@ -350,6 +340,8 @@ def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.n
         np.asfortranarray(slater, np.complex128).ravel(order="F").tofile(f)
         np.array([global_update_loop], np.int32).tofile(f)
         np.array([global_distance * model.char_length], np.float64).tofile(f)
+        np.array([nbins, nmeas_per_bin, meas_number, warmup, meas_interval, checkpoint_loop, int(bool(resume))],
+                 np.int32).tofile(f)
 
 
 def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:

@@ -30,7 +30,7 @@ def test_driver_input_layout_round_trip(tmp_path):
     host.write_driver_input(str(path), m, x, sl, nsweeps=2, seed=1234)
     npla, Ns, Nb, Lt, ne = m.npla, m.Ns, m.Nb, m.p.Lt, m.p.ne
     expect = (8 * 4 + 4 * npla * 4 * 4 + 2 * 4 * npla * 4 + 4 * npla * 16 * 16 + (4 + 4 + 16 + 16 + 2) * 16
-              + 7 * 8 + 2 * 4 + Nb * Lt * 8 + Ns * ne * 16 + 4 + 8)
+              + 7 * 8 + 2 * 4 + Nb * Lt * 8 + Ns * ne * 16 + 4 + 8 + 7 * 4)
     assert path.stat().st_size == expect
 
 
