@@ -8,6 +8,11 @@ iteration of the reference's sweep loop (all slices x 4 bond families x all
 bonds, wraps, QR stabilization every 10 slices, one measurement at Lt/2).
 Directions alternate starting backward, as in the reference.
 
+`batched` reports the batched form: --batch chains of the same workload
+advanced together (pqmc_sweep_batch), their aggregate chain-sweeps/s and the
+largest |G - G_plain| of the batch's first chain, which replays the plain
+chain's fields and draws.
+
 Prints one JSON line.  `roofline` is that of the dominant kernel class,
 timed with HIP events on the chain's own stream over the timed region;
 `cpu_baseline` times the oracle (a single-threaded C restatement of the
@@ -73,6 +78,42 @@ def cpu_baseline(L, Lt, ng, ne, seed, target_s):
                       f"({dt:.1f} s), scaled by slices"}
 
 
+def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_plain):
+    """K chains of the workload, one batch sweep per step; chain 0 replays the
+    plain chain (same fields, trial and draws), chains i > 0 take fields and
+    draws from seed + i."""
+    from pqmc_allinone_amd.host import Chain, Ran2, sweep_batch
+    from pqmc_allinone_amd.model import synthetic_inputs
+    chains, rngs = [], []
+    for i in range(K):
+        xi = x0 if i == 0 else synthetic_inputs(m, seed + i)[0]
+        ch = Chain(m, block_bonds=block_bonds, helper_bonds=helper_bonds)
+        ch.set_fields(xi)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        chains.append(ch)
+        rngs.append(Ran2(seed + i))
+    nd = 2 * m.Nb * m.p.Lt
+    fw = True
+    for _ in range(warmup):
+        fw = not fw
+        sweep_batch(chains, [fw] * K, [r.draws(nd) for r in rngs], measure=True)
+    for ch in chains:
+        ch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fw = not fw
+        sweep_batch(chains, [fw] * K, [r.draws(nd) for r in rngs], measure=True)
+    for ch in chains:
+        ch.synchronize()
+    dt = time.perf_counter() - t0
+    g_err = float(np.abs(chains[0].g_h - g_plain).max())
+    for ch in chains:
+        ch.close()
+    return {"chains": K, "helper_bonds": helper_bonds, "value": K * steps / dt, "unit": "chain-sweeps/s",
+            "ms_per_step": 1e3 * dt / steps, "dtype": "c128 (fp64)", "max_abs_g_err_vs_plain": g_err}
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -97,10 +138,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--batch", type=int, default=5,
+                    help="chains of the batched leg (0 or 1: no batched leg)")
+    ap.add_argument("--batch-helper-bonds", type=int, default=2,
+                    help="bonds per column-owning workgroup in the batched leg's family passes (1 or 2)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
     args = ap.parse_args()
+    if args.batch > 1:
+        # one hardware queue per chain of the batched leg (read when the HIP
+        # runtime starts, so before the first device call)
+        q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        if q < args.batch + 2:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.batch + 3))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -166,6 +217,11 @@ def main():
         roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None,
                 "flops_per_launch": fl}
+    batched = None
+    if args.batch > 1:
+        batched = batched_leg(m, x, sl, seed, args.batch, args.warmup, args.steps, args.block_bonds,
+                              args.batch_helper_bonds, ch.g_h)
+        batched["vs_plain"] = batched["value"] / sps
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
@@ -189,6 +245,7 @@ def main():
                            "frac_fp64": F * sps / PEAK_FP64},
         "acceptance": n_acc / max(n_prop, 1),
         "kernels": kern,
+        "batched": batched,
         "cpu_baseline": cpu,
         "speedup_vs_cpu_core": (sps / cpu["value"]) if cpu else None,
     }

@@ -41,6 +41,9 @@ typedef struct pqmc_config {
   int ngroup;         /* stabilization interval */
   int ne;             /* nelec */
   int block_bonds;    /* bonds per decision block of a family pass (1..8), 0 = automatic */
+  int helper_bonds;   /* bonds per column-owning workgroup of a family pass (1 or 2), 0 = automatic:
+                         1 when the pass fits the device.  2 halves the compute units a pass keeps
+                         resident, so that more chains fit a batch (pqmc_sweep_batch). */
 } pqmc_config;
 
 /* Model tables as built by set_typed_pf / init_pf_para
@@ -109,6 +112,17 @@ int pqmc_measure(pqmc_handle *h, int time, double *out);
  * time_count iterations (0 = all).  meas_out (total_width complex) may be
  * NULL.  Accept flags stay on the device (pqmc_get_accepts). */
 int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, int measure, double *meas_out);
+
+/* n chains, one sweep each, advanced at the same time (the batched form):
+ * one host thread per chain runs pqmc_sweep(hs[i], forward[i], u[i], 0,
+ * measure, meas_out[i]) on the chain's own stream.  Chains may differ in
+ * lattice and Lt.  The family passes of all n chains must be resident on the
+ * device together; a batch that does not fit is refused with PQMC_EINVAL.
+ * meas_out may be NULL (or hold NULL entries).  Returns the first failing
+ * chain's status.  No reference counterpart: the reference runs one chain
+ * per MPI rank (src/Main_PQMC.f90:380-400). */
+int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forward, const float *const *u, int measure,
+                     double *const *meas_out);
 
 /* State readers/writers (parity and checkpointing). */
 int pqmc_get_g(pqmc_handle *h, double *g);

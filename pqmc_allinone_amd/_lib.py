@@ -23,7 +23,7 @@ class PQMCError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("L1", C.c_int), ("L2", C.c_int), ("Lt", C.c_int), ("ngroup", C.c_int), ("ne", C.c_int),
-                ("block_bonds", C.c_int)]
+                ("block_bonds", C.c_int), ("helper_bonds", C.c_int)]
 
 
 class ModelTables(C.Structure):
@@ -57,6 +57,7 @@ SIGNATURES = {
     "pqmc_g_scratch": (_I, [_P, _I, _I, _P]),
     "pqmc_measure": (_I, [_P, _I, _P]),
     "pqmc_sweep": (_I, [_P, _I, _P, _I, _I, _P]),
+    "pqmc_sweep_batch": (_I, [_P, _I, _P, _P, _I, _P]),
     "pqmc_get_g": (_I, [_P, _P]),
     "pqmc_set_g": (_I, [_P, _P]),
     "pqmc_get_qstring": (_I, [_P, _I, _P]),

@@ -7,6 +7,8 @@
 //    (src/mod_matrixlib.f90:452-467).
 //  * k_trsm_upper: Y = R^-1 Z, used for (L R)^-1 through the QR of L R.
 //  * small reductions: log-weight, max |G_fast - G| (err_fast).
+#include <mutex>
+
 #include "pqmc_common.h"
 
 // ---------------------------------------------------------------------------
@@ -279,11 +281,15 @@ extern "C" hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V
                                     zd *tau, double *vn2, zd *beta_out, hipStream_t s) {
   if (m - j0 > 1024 || nb > 16) return hipErrorInvalidValue;
   const size_t lds = sizeof(zd) * (size_t)(m - j0) * nb;
+  static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 48 * 1024;
-  if (lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_qr_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    attr = lds;
+  {
+    std::lock_guard<std::mutex> lk(attr_mu);
+    if (lds > attr) {
+      hipError_t e = hipFuncSetAttribute((const void *)k_qr_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = lds;
+    }
   }
   hipLaunchKernelGGL(k_qr_panel, dim3(1), dim3(QR_THREADS), lds, s, A, lda, m, j0, nb, V, ldv, T, tau, vn2,
                      beta_out);
@@ -542,11 +548,15 @@ extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ld
   const int nblk = (n + TS_KB - 1) / TS_KB;
   hipLaunchKernelGGL(k_trinv_diag, dim3(nblk), dim3(64), 0, s, R, ldr, n, work);
   const size_t lds = sizeof(zd) * (size_t)n * (TS_COLS + TS_KB);
+  static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 48 * 1024;
-  if (lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    attr = lds;
+  {
+    std::lock_guard<std::mutex> lk(attr_mu);
+    if (lds > attr) {
+      hipError_t e = hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = lds;
+    }
   }
   hipLaunchKernelGGL(k_trsm_upper, dim3((ncols + TS_COLS - 1) / TS_COLS), dim3(256), lds, s, R, ldr, n, Z, ldz,
                      ncols, (const zd *)work);

@@ -40,6 +40,8 @@
 // (cal_prob, :475-500).  As log u < 0 this is exactly w < log u, i.e.
 // |det|^2 < exp(2 (log u - dS) / (ncopy*(1+TR))); the right-hand side does
 // not depend on G and is prepared before the bond's turn.
+#include <mutex>
+
 #include "pqmc_common.h"
 
 #include "k_pass.h"
@@ -134,6 +136,9 @@ static __device__ unsigned long long ps_wait(const ps_args &A, unsigned long lon
     __builtin_amdgcn_s_sleep(1);
   }
 }
+
+// helpers that have finished the backward wrap at the start of the pass
+static __device__ __forceinline__ unsigned long long *ps_wrapped(const ps_args &A) { return &A.sync[3]; }
 
 // count of the owners of block kstep+1+mb that have published after step kstep
 static __device__ __forceinline__ unsigned long long *ps_pub(const ps_args &A, int kstep, int mb) {
@@ -472,6 +477,9 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
   PROF_T0();
   int iW = 0, i10 = 1, i01 = 2, i11 = 3;
   bool ok = true;
+  // a backward pass wraps every column with the exponentials before the
+  // update; none may change until all helpers have read them
+  bool ekv_free = A.wrap != 2;
   // proposals of block 0; the window after the helpers' load
   if (tid < min(nb, npla)) ps_propose(A, tid, L.prop[0][tid]);
   if (tid == 0) {
@@ -574,6 +582,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
           L.li[m] = j;
           L.dets[m] = zmk(dre, dim);
           A.x[fld + tbase] = xn;
+          if (!ekv_free) ekv_free = ps_wait(A, ps_wrapped(A), (unsigned long long)(gridDim.x - 1), nullptr, 0) != 0;
           st_coh_ekv(&A.ekv[((size_t)(A.t - 1) * 4 + A.f) * npla + p0 + j], en);
         }
         wave_sync();
@@ -924,7 +933,11 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
   ps_publish_cols(A, sC, sS, b0, b1, -1);
   ps_release();
   __syncthreads();
-  if (tid == 0) ps_announce(A, b0, b1, -1);
+  if (tid == 0) {
+    ps_announce(A, b0, b1, -1);
+    // this workgroup has read the exponentials its backward wrap needs
+    atomicAdd(ps_wrapped(A), 1ull);
+  }
   PROF_ACC(0);
   for (int k = 0; k < A.nblk; ++k) {
     // Block k's columns are in the slot once their owners have published
@@ -1052,6 +1065,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     const unsigned long long done = atomicAdd(&A.sync[1], 1ull);
     if (done == gridDim.x - 1) {
       ps_st(&A.sync[2], 0ull);
+      ps_st(ps_wrapped(A), 0ull);
       for (int k = -1; k < A.nblk; ++k) {
         ps_st(ps_pub(A, k, 0), 0ull);
         ps_st(ps_pub(A, k, 1), 0ull);
@@ -1073,11 +1087,15 @@ extern "C" size_t pqmc_pass_rec_bytes(void) { return sizeof(ps_rec); }
 extern "C" hipError_t pqmc_launch_pass(const ps_args *a, hipStream_t s) {
   if (a->bph < 1 || a->bph > 2) return hipErrorInvalidValue;
   const size_t lds = pqmc_pass_lds(a->md.Ns, a->bph);
+  static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 0;
-  if (lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_family_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    attr = lds;
+  {
+    std::lock_guard<std::mutex> lk(attr_mu);
+    if (lds > attr) {
+      hipError_t e = hipFuncSetAttribute((const void *)k_family_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = lds;
+    }
   }
   hipLaunchKernelGGL(k_family_pass, dim3(a->H + 1), dim3(PS_THREADS), lds, s, *a);
   return hipGetLastError();

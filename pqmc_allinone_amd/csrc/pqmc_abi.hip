@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pqmc_hip.h"
@@ -84,6 +85,7 @@ struct ev_pool {
 struct pqmc_handle {
   int L1, L2, Ncell, Ns, ne, Nb, npla, Lt, ngroup, nblock, total_width, nbq;
   int nb, nblk, bph, H;  // family pass: bonds per decision block, blocks, bonds per helper, helpers
+  int dev = 0;           // device the chain lives on
   model_t md{};
   hipStream_t st = nullptr;
   bool model_set = false, trial_set = false, fields_set = false, g_ready = false;
@@ -170,8 +172,8 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
     return fail(PQMC_EINVAL, "invalid sizes");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PQMC_ENODEV, "no HIP device");
+  int dev = 0;
   {
-    int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
       return fail(PQMC_ENODEV, "cannot query the current HIP device");
@@ -179,6 +181,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
       return fail(PQMC_ENODEV, std::string("current device is ") + prop.gcnArchName + ", the library is built for gfx950");
   }
   pqmc_handle *h = new pqmc_handle();
+  h->dev = dev;
   h->L1 = cfg->L1;
   h->L2 = cfg->L2;
   h->Ncell = cfg->L1 * cfg->L2;
@@ -198,11 +201,13 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   {
     // every workgroup of a family pass must be resident at once: one leader
     // plus ceil(npla / bph) helpers
+    if (cfg->helper_bonds < 0 || cfg->helper_bonds > 2) { delete h; return fail(PQMC_EINVAL, "helper_bonds out of range"); }
     h->bph = 0;
-    for (int bph = 1; bph <= 2 && !h->bph; ++bph) {
+    for (int bph = cfg->helper_bonds ? cfg->helper_bonds : 1; bph <= 2 && !h->bph; ++bph) {
       int maxg = 0;
       if (pqmc_pass_max_groups(h->Ns, bph, &maxg) != hipSuccess) { delete h; return fail(PQMC_EHIP, "occupancy query failed"); }
       if ((h->npla + bph - 1) / bph + 1 <= maxg) h->bph = bph;
+      if (cfg->helper_bonds) break;
     }
     if (!h->bph) { delete h; return fail(PQMC_EINVAL, "lattice too large for a resident family pass"); }
     h->H = (h->npla + h->bph - 1) / h->bph;
@@ -760,6 +765,54 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass");
+  return PQMC_OK;
+}
+
+// Several chains, one sweep each, at the same time: one host thread per
+// chain drives pqmc_sweep on the chain's own stream.  A family pass spins
+// until all of its workgroups are resident, so the passes of every chain of
+// the batch must fit the device together; the batch is refused otherwise.
+extern "C" int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forward, const float *const *u,
+                                int measure, double *const *meas_out) {
+  if (!hs || !forward || !u || n < 1) return fail(PQMC_EINVAL, "null argument or empty batch");
+  int ncu = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!hs[i] || !u[i]) return fail(PQMC_EINVAL, "null handle or draws in the batch");
+    for (int j = 0; j < i; ++j)
+      if (hs[j] == hs[i]) return fail(PQMC_EINVAL, "a chain appears twice in the batch");
+    if (hs[i]->dev != hs[0]->dev) return fail(PQMC_EINVAL, "the chains of a batch must share one device");
+  }
+  HIPCHK(hipSetDevice(hs[0]->dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hs[0]->dev));
+  long need = 0;  // compute units the passes occupy together
+  for (int i = 0; i < n; ++i) {
+    int maxg = 0;
+    HIPCHK(pqmc_pass_max_groups(hs[i]->Ns, hs[i]->bph, &maxg));
+    const int per_cu = maxg / (ncu > 0 ? ncu : 1);
+    if (per_cu < 1) return fail(PQMC_EINVAL, "a family pass workgroup does not fit one compute unit");
+    need += (hs[i]->H + 1 + per_cu - 1) / per_cu;
+  }
+  if (need > ncu)
+    return fail(PQMC_EINVAL, "the batch's family passes (" + std::to_string(need) + " compute units) do not fit the " +
+                                 std::to_string(ncu) + " compute units of the device together");
+  std::vector<int> rc(n, PQMC_OK);
+  std::vector<std::string> msg(n);
+  auto one = [&](int i) {
+    if (hipSetDevice(hs[i]->dev) != hipSuccess) {
+      rc[i] = PQMC_EHIP;
+      msg[i] = "hipSetDevice failed";
+      return;
+    }
+    rc[i] = pqmc_sweep(hs[i], forward[i], u[i], 0, measure, meas_out ? meas_out[i] : nullptr);
+    if (rc[i]) msg[i] = g_err;
+  };
+  std::vector<std::thread> th;
+  th.reserve(n - 1);
+  for (int i = 1; i < n; ++i) th.emplace_back(one, i);
+  one(0);
+  for (auto &t : th) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rc[i]) return fail(rc[i], "chain " + std::to_string(i) + " of the batch: " + msg[i]);
   return PQMC_OK;
 }
 

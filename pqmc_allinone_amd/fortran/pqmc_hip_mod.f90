@@ -11,6 +11,7 @@ module pqmc_hip
 
   type, bind(C) :: pqmc_config
     integer(c_int) :: L1, L2, Lt, ngroup, ne, block_bonds
+    integer(c_int) :: helper_bonds = 0
   end type pqmc_config
 
   type, bind(C) :: pqmc_model
@@ -118,6 +119,17 @@ module pqmc_hip
       real(c_float), intent(in) :: u(*)
       type(c_ptr), value :: meas_out
     end function pqmc_sweep
+
+    ! hs, u and meas_out are arrays of c_ptr (c_loc of each chain's arrays)
+    integer(c_int) function pqmc_sweep_batch(hs, n, forward, u, measure, meas_out) &
+        bind(C, name='pqmc_sweep_batch')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int), value :: n, measure
+      integer(c_int), intent(in) :: forward(*)
+      type(c_ptr), intent(in) :: u(*)
+      type(c_ptr), value :: meas_out
+    end function pqmc_sweep_batch
 
     integer(c_int) function pqmc_get_g(h, g) bind(C, name='pqmc_get_g')
       import :: c_int, c_ptr, c_double_complex

@@ -147,11 +147,11 @@ def propose_global(model: Model, x: np.ndarray, kind: str, rng: "Ran2", distance
 class Chain:
     """One projector-DQMC Markov chain resident on the GPU."""
 
-    def __init__(self, model: Model, block_bonds: int = 0):
+    def __init__(self, model: Model, block_bonds: int = 0, helper_bonds: int = 0):
         self.model = model
         p = model.p
         self._L = _lib.lib()
-        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(block_bonds))
+        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(block_bonds), int(helper_bonds))
         h = C.c_void_p()
         _lib.check(self._L.pqmc_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -357,6 +357,28 @@ def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
     return {"accepts": acc, "g": g, "fields": fields, "meas_sum": meas, "n_meas": n_meas,
             "positive_accept": int(cnt[0]), "negative_accept": int(cnt[1]),
             "global_accept": int(cnt[2]), "global_reject": int(cnt[3])}
+
+
+def sweep_batch(chains, forwards, draws, measure: bool = False):
+    """One sweep of every chain, at the same time (pqmc_sweep_batch): one
+    host thread per chain, each on its chain's stream.  Returns the
+    measurements (one total_width vector per chain) when measure is set."""
+    n = len(chains)
+    if n < 1 or len(forwards) != n or len(draws) != n:
+        raise ValueError("one direction and one draw array per chain")
+    us = []
+    for ch, d in zip(chains, draws):
+        u = np.ascontiguousarray(d, np.float32)
+        if u.size != 2 * ch.Nb * ch.model.p.Lt:
+            raise ValueError("need 2*Nb*Lt draws per chain")
+        us.append(u)
+    hs = (C.c_void_p * n)(*[ch.h.value for ch in chains])
+    fw = (C.c_int * n)(*[int(bool(f)) for f in forwards])
+    up = (C.c_void_p * n)(*[u.ctypes.data for u in us])
+    outs = [np.empty(ch.total_width, np.complex128) for ch in chains] if measure else None
+    op = (C.c_void_p * n)(*[o.ctypes.data for o in outs]) if measure else None
+    _lib.check(_lib.lib().pqmc_sweep_batch(hs, n, fw, up, int(bool(measure)), op))
+    return outs
 
 
 class GlobalMoves:

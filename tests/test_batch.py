@@ -1,0 +1,73 @@
+"""The batched form (pqmc_sweep_batch): chains of uneven lattice and Lt
+advanced together, each against the oracle run alone on the same inputs and
+draws: identical accepts, G and measurements to 1e-10.  A batch whose family
+passes cannot be resident together is refused before anything is launched."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pqmc_allinone_amd._lib import PQMCError
+from pqmc_allinone_amd.host import Chain, Ran2, sweep_batch
+from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.gpu
+def test_batch_of_uneven_chains_matches_oracle():
+    cases = [(4, 100, 10, 1234), (6, 60, 10, 99), (4, 100, 10, 55)]
+    chains, orcs, rngs, models = [], [], [], []
+    for L, Lt, ng, seed in cases:
+        m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=L * L))
+        x, sl = synthetic_inputs(m, seed)
+        ch = Chain(m)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        o = O.Oracle(L, L, Lt, ng, L * L)
+        o.set_fields(x)
+        o.set_trial(sl)
+        o.init(1)
+        chains.append(ch)
+        orcs.append(o)
+        rngs.append(Ran2(seed))
+        models.append(m)
+    forward = True
+    for sweep in range(3):
+        forward = not forward
+        draws = [r.draws(2 * m.Nb * m.p.Lt) for r, m in zip(rngs, models)]
+        print(f"batch sweep {sweep} forward={forward}", flush=True)
+        outs = sweep_batch(chains, [forward] * 3, draws, measure=True)
+        for i, (ch, o) in enumerate(zip(chains, orcs)):
+            acc_o, mo = o.sweep(forward, draws[i], measure=True)
+            assert np.array_equal(ch.accepts(), acc_o), (sweep, i)
+            assert np.array_equal(ch.get_fields(), o.get_fields()), (sweep, i)
+            assert rel(ch.g_h, o.g) < 1e-10, (sweep, i)
+            assert rel(outs[i], mo) < 1e-10, (sweep, i)
+    for ch in chains:
+        ch.close()
+
+
+@pytest.mark.gpu
+def test_batch_that_does_not_fit_is_refused():
+    """Three 14x14 chains need 3 x 99 resident family-pass workgroups of one
+    compute unit each, more than the device has; the host check refuses the
+    batch before any launch."""
+    m = build_model(ModelParams(L1=14, L2=14, Lt=20, ngroup=10, ne=196))
+    x, sl = synthetic_inputs(m, 3)
+    chains = []
+    for _ in range(3):
+        ch = Chain(m)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        chains.append(ch)
+    u = np.zeros(2 * m.Nb * m.p.Lt, np.float32)
+    with pytest.raises(PQMCError, match="compute units"):
+        sweep_batch(chains, [False] * 3, [u] * 3)
+    with pytest.raises(PQMCError, match="twice"):
+        sweep_batch([chains[0], chains[0]], [False] * 2, [u] * 2)
+    for ch in chains:
+        ch.close()
