@@ -44,6 +44,11 @@ struct orc_model {
   zc *g;                   /* g_h(Ns,Ns) */
   zc *Q, *Dst, *R;         /* Q_string(Ns,ne,nblock), D_string(ne,nblock), R_string */
   zc *slQ, *slD, *slR;
+  /* finite temperature (proj = .false., src/mod_evolution.f90:411-785):
+   * block b holds a right string B(t,0) = Q diag(D) T (T unit upper
+   * products) or a left string B(beta,t) = T diag(D) Q (Q rows orthonormal) */
+  int ft;
+  zc *FQ, *FD, *FT; /* Ns x Ns x nblock, Ns x nblock, Ns x Ns x nblock */
   zc ln_cw;
   double err_fast, err_fast_max;
   long pos_acc, neg_acc;
@@ -468,6 +473,7 @@ void orc_destroy(orc_model *m) {
   free(m->sites); free(m->bfi); free(m->cross); free(m->bcph);
   free(m->x); free(m->ekv); free(m->ekvi); free(m->g);
   free(m->Q); free(m->Dst); free(m->R); free(m->slQ); free(m->slD); free(m->slR);
+  free(m->FQ); free(m->FD); free(m->FT);
   free(m);
 }
 
@@ -649,8 +655,151 @@ void orc_set_trial(orc_model *m, const double *slater) {
   zqdr(m->Ns, m->ne, m->slQ, m->slR, m->slD);
 }
 
+/* ------------------------------------------------------------------ */
+/* finite temperature (src/mod_evolution.f90:411-785)                 */
+/* ------------------------------------------------------------------ */
+/* The reference's FT branch is marked "not used for now" and is not
+ * self-consistent (init_g_FT's active call forms (1 + B B^H)^-1, the
+ * commented call beside it (1 + B)^-1; the left string's unitary and
+ * triangular factors are stored in swapped roles), so this restates the
+ * quantities it is built to compute, with its structure: right strings
+ * B(t,0) = Q D T from column-scaled QR (Q' = Q*D, then a UDV step), left
+ * strings B(beta,t) = T D Q from row-scaled LQ, and G from the scale-split
+ * formula of cal_g_and_det_FT (:494-648).  The reference's UDV is an SVD
+ * (ZGESVD, src/mod_matrixlib.f90:704-743); a QR-based UDV gives the same G
+ * and the same Re ln det(1 + B), which is all that enters the sweep. */
+
+static zc *FQS(orc_model *m, int b) { return m->FQ + (size_t)m->Ns * m->Ns * (b - 1); }
+static zc *FDS(orc_model *m, int b) { return m->FD + (size_t)m->Ns * (b - 1); }
+static zc *FTS(orc_model *m, int b) { return m->FT + (size_t)m->Ns * m->Ns * (b - 1); }
+
+void orc_set_ft(orc_model *m, int on) {
+  m->ft = on ? 1 : 0;
+  if (m->ft && !m->FQ) {
+    size_t n = (size_t)m->Ns * m->Ns * m->nblock;
+    m->FQ = xcalloc(n, sizeof(zc));
+    m->FT = xcalloc(n, sizeof(zc));
+    m->FD = xcalloc((size_t)m->Ns * m->nblock, sizeof(zc));
+  }
+}
+
+static void ft_identity(orc_model *m, int b) {
+  int Ns = m->Ns;
+  memset(FQS(m, b), 0, sizeof(zc) * Ns * Ns);
+  memset(FTS(m, b), 0, sizeof(zc) * Ns * Ns);
+  for (int i = 0; i < Ns; ++i) {
+    FQS(m, b)[i + (size_t)Ns * i] = 1;
+    FTS(m, b)[i + (size_t)Ns * i] = 1;
+    FDS(m, b)[i] = 1;
+  }
+}
+
+/* right string of block b from block b-1 and the slices t0..t1 */
+static void ft_right(orc_model *m, int b, int t0, int t1) {
+  int Ns = m->Ns;
+  zc *x = xcalloc((size_t)Ns * Ns, sizeof(zc)), *r = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  memcpy(x, FQS(m, b - 1), sizeof(zc) * Ns * Ns);
+  for (int p = t0; p <= t1; ++p)
+    for (int f = 1; f <= 4; ++f) orc_left_evolve(m, p, (double *)x, Ns, f, 0);
+  for (int j = 0; j < Ns; ++j)
+    for (int i = 0; i < Ns; ++i) x[i + (size_t)Ns * j] *= FDS(m, b - 1)[j]; /* Q' = Q*D */
+  zqdr(Ns, Ns, x, r, FDS(m, b));
+  memcpy(FQS(m, b), x, sizeof(zc) * Ns * Ns);
+  gemm_nn(Ns, Ns, Ns, r, FTS(m, b - 1), FTS(m, b)); /* T_b = T' T_{b-1} */
+  free(r);
+  free(x);
+}
+
+/* left string of block b from block b+1 and the slices t1 down to t0 */
+static void ft_left(orc_model *m, int b, int t0, int t1) {
+  int Ns = m->Ns;
+  zc *y = xcalloc((size_t)Ns * Ns, sizeof(zc)), *l = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  memcpy(y, FQS(m, b + 1), sizeof(zc) * Ns * Ns);
+  for (int p = t1; p >= t0; --p)
+    for (int f = 4; f >= 1; --f) orc_right_evolve(m, p, (double *)y, Ns, f, 0);
+  for (int j = 0; j < Ns; ++j)
+    for (int i = 0; i < Ns; ++i) y[i + (size_t)Ns * j] *= FDS(m, b + 1)[i]; /* rows scaled by D */
+  zldq(Ns, Ns, y, l, FDS(m, b));
+  memcpy(FQS(m, b), y, sizeof(zc) * Ns * Ns);
+  gemm_nn(Ns, Ns, Ns, FTS(m, b + 1), l, FTS(m, b)); /* T_b = T_{b+1} T' */
+  free(l);
+  free(y);
+}
+
+/* G = (1 + Q_r D_r T_r T_l D_l Q_l)^-1 with D split at |d| = 1
+ * (cal_g_and_det_FT, src/mod_evolution.f90:494-648); g_h = 1 - G;
+ * ln_cw = ln det(1 + B(beta,0)) */
+static void ft_cal_g(orc_model *m, int br, int bl) {
+  int Ns = m->Ns;
+  const zc *Qr = FQS(m, br), *Dr = FDS(m, br), *Tr = FTS(m, br);
+  const zc *Ql = FQS(m, bl), *Dl = FDS(m, bl), *Tl = FTS(m, bl);
+  zc *dbr = xcalloc(Ns, sizeof(zc)), *dsr = xcalloc(Ns, sizeof(zc));
+  zc *dbl = xcalloc(Ns, sizeof(zc)), *dsl = xcalloc(Ns, sizeof(zc));
+  for (int i = 0; i < Ns; ++i) {
+    if (cabs(Dr[i]) > 1.0) { dbr[i] = Dr[i]; dsr[i] = 1; } else { dbr[i] = 1; dsr[i] = Dr[i]; }
+    if (cabs(Dl[i]) > 1.0) { dbl[i] = Dl[i]; dsl[i] = 1; } else { dbl[i] = 1; dsl[i] = Dl[i]; }
+  }
+  zc *qq = xcalloc((size_t)Ns * Ns, sizeof(zc)), *tt = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  zc *mm = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  gemm_nn(Ns, Ns, Ns, Ql, Qr, qq); /* (Q_r^H Q_l^H) = (Q_l Q_r)^H */
+  gemm_nn(Ns, Ns, Ns, Tr, Tl, tt);
+  for (int j = 0; j < Ns; ++j)
+    for (int i = 0; i < Ns; ++i)
+      mm[i + (size_t)Ns * j] = conj(qq[j + (size_t)Ns * i]) / dbr[i] / dbl[j] + dsr[i] * tt[i + (size_t)Ns * j] * dsl[j];
+  zc lw = clog(zdet(Ns, mm));
+  zc lq = clog(zdet(Ns, Qr)) + clog(zdet(Ns, Ql));
+  for (int i = 0; i < Ns; ++i) lw += clog(dbr[i]) + clog(dbl[i]);
+  lw += lq;
+  zinverse(Ns, mm);
+  /* G = Q_l^H Db_l^-1 M^-1 Db_r^-1 Q_r^H */
+  for (int j = 0; j < Ns; ++j)
+    for (int i = 0; i < Ns; ++i) mm[i + (size_t)Ns * j] = mm[i + (size_t)Ns * j] / dbl[i] / dbr[j];
+  zc *qlh = xcalloc((size_t)Ns * Ns, sizeof(zc)), *qrh = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  conj_transpose(Ns, Ns, Ql, qlh);
+  conj_transpose(Ns, Ns, Qr, qrh);
+  gemm_nn(Ns, Ns, Ns, qlh, mm, tt);
+  gemm_nn(Ns, Ns, Ns, tt, qrh, m->g);
+  for (size_t e = 0; e < (size_t)Ns * Ns; ++e) m->g[e] = -m->g[e];
+  for (int i = 0; i < Ns; ++i) m->g[i + (size_t)Ns * i] += 1;
+  if (m->tr_weight) lw = lw + conj(lw);
+  m->ln_cw = ln_branch(lw * (double)m->ncopy);
+  free(qlh); free(qrh); free(qq); free(tt); free(mm);
+  free(dbr); free(dsr); free(dbl); free(dsl);
+}
+
+/* init_g_FT: right strings of blocks 2..nblock-1 from the identity at block
+ * 1, the identity as the left string at block nblock, G at ntime+1 */
+static void ft_init(orc_model *m, int start_time) {
+  int ng = m->ngroup, nb = m->nblock;
+  (void)start_time;
+  ft_identity(m, 1);
+  ft_identity(m, nb);
+  for (int b = 2; b <= nb - 1; ++b) ft_right(m, b, (b - 2) * ng + 1, (b - 1) * ng);
+  ft_cal_g(m, nb - 1, nb);
+}
+
+/* get_g_scratch_FT (src/mod_evolution.f90:651-740) */
+static void ft_scratch(orc_model *m, int time, int forward) {
+  int ng = m->ngroup;
+  if (forward) {
+    int block = time / ng + 1;
+    ft_right(m, block, time - ng, time - 1);
+    ft_cal_g(m, block, block + 1);
+  } else {
+    int block = time / ng + 2;
+    if (ng == 1) block -= 1;
+    ft_left(m, block, time, time + ng - 1);
+    ft_cal_g(m, block - 1, block);
+  }
+}
+
 /* init_g_T0 of src/mod_evolution.f90:46-111 */
 void orc_init_g(orc_model *m, int start_time) {
+  if (m->ft) {
+    m->ln_cw = 0;
+    ft_init(m, start_time);
+    return;
+  }
   int Ns = m->Ns, ne = m->ne, ng = m->ngroup, nb = m->nblock;
   m->ln_cw = 0;
   memcpy(QS(m, 1), m->slQ, sizeof(zc) * Ns * ne);
@@ -683,6 +832,23 @@ void orc_get_g_scratch(orc_model *m, int time, int forward, double *err_out) {
   int Ns = m->Ns, ne = m->ne, ng = m->ngroup;
   zc ln_fast = m->ln_cw;
   m->ln_cw = 0;
+  if (m->ft) {
+    zc *gf = xcalloc((size_t)Ns * Ns, sizeof(zc));
+    memcpy(gf, m->g, sizeof(zc) * Ns * Ns);
+    ft_scratch(m, time, forward);
+    double err = 0;
+    for (size_t i = 0; i < (size_t)Ns * Ns; ++i) {
+      double v = cabs(gf[i] - m->g[i]);
+      if (v > err) err = v;
+    }
+    m->err_fast = err;
+    if (err > m->err_fast_max) m->err_fast_max = err;
+    double e2 = fabs(creal(m->ln_cw - ln_fast));
+    if (e2 > m->err_fast_max) m->err_fast_max = e2;
+    if (err_out) *err_out = err;
+    free(gf);
+    return;
+  }
   zc *rmat = xcalloc((size_t)Ns * ne, sizeof(zc));
   zc *lmat = xcalloc((size_t)Ns * ne, sizeof(zc));
   zc *gfast = xcalloc((size_t)Ns * Ns, sizeof(zc));

@@ -64,6 +64,9 @@ void orc_get_expkv(const orc_model *m, int pf_id, int pla, double x, int inv, do
 
 /* G and B-string */
 void orc_init_g(orc_model *m, int start_time);
+/* finite temperature (proj = .false.): init_g and get_g_scratch build
+ * Ns x Ns strings and G = (1 + B(tau,0) B(beta,tau))^-1 (g_h = 1 - G) */
+void orc_set_ft(orc_model *m, int on);
 void orc_get_g(const orc_model *m, double *g_c128);
 void orc_set_g(orc_model *m, const double *g_c128);
 void orc_get_qstring(const orc_model *m, int block, double *q_c128);

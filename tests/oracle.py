@@ -52,6 +52,7 @@ def lib():
         L.orc_get_qstring.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_get_expkv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p]
         L.orc_init_g.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_ft.argtypes = [C.c_void_p, C.c_int]
         L.orc_left_evolve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_right_evolve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_wrap_g.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -111,7 +112,7 @@ class Oracle:
     """One oracle chain (the reference's module state on the CPU)."""
 
     def __init__(self, L1, L2, Lt, ngroup, ne, dtau=0.1, lam=0.64, omega=1.0, jump=1.0, hop=1.0,
-                 bc_phase=(0.0, 0.0), biased=0.0, ncopy=1, tr_weight=False):
+                 bc_phase=(0.0, 0.0), biased=0.0, ncopy=1, tr_weight=False, ft=False):
         self.p = Params(L1, L2, Lt, ngroup, ne, dtau, lam, omega, jump, hop, bc_phase[0], bc_phase[1], biased,
                         ncopy, int(tr_weight))
         self.m = lib().orc_create(C.byref(self.p))
@@ -121,6 +122,8 @@ class Oracle:
         self.Ns, self.Nb, self.npla, self.nblock = d.Ns, d.Nb, d.npla, d.nblock
         self.Lt, self.ne, self.ngroup = Lt, ne, ngroup
         self.total_width = d.total_width
+        if ft:
+            lib().orc_set_ft(self.m, 1)
 
     def __del__(self):
         try:
