@@ -44,6 +44,10 @@ typedef struct pqmc_config {
   int helper_bonds;   /* bonds per column-owning workgroup of a family pass (1 or 2), 0 = automatic:
                          1 when the pass fits the device.  2 halves the compute units a pass keeps
                          resident, so that more chains fit a batch (pqmc_sweep_batch). */
+  int finite_temperature;  /* 0: projector (proj = .true.); 1: finite temperature, G = (1 + B(beta,0))^-1
+                              from Ns x Ns UDT strings (init_g_FT / get_g_scratch_FT,
+                              src/mod_evolution.f90:411-785).  Needs Lt % ngroup == 0; no trial
+                              matrix; pqmc_global_weight and the T0 string readers refuse it. */
 } pqmc_config;
 
 /* Model tables as built by set_typed_pf / init_pf_para

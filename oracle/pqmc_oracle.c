@@ -1313,10 +1313,10 @@ void orc_ran2_fill(orc_ran2 *s, int64_t n, float *out) {
 
 /* zlq of src/mod_matrixlib.f90:488-503 (ZGELQF + ZUNGLQ): a(d x n) = L Q,
  * a <- Q, l (d x d) the lower factor.  ZGELQF reflects the conjugated rows
- * with zlarfg, which is zgeqr2 on a^H: L = R^H with the same real diagonal
- * and Q = Q'^H.  The diagonal keeps zlarfg's (beta, +0): a negative entry
- * then has log = log|beta| + i pi as in the reference, not the -i pi that a
- * conjugated -0 imaginary part would give. */
+ * with zlarfg, which is zgeqr2 on a^H: L = R^H and Q = Q'^H.  ZGELQ2
+ * conjugates each row back after zlarfg, the diagonal included, so the
+ * diagonal is conj((beta, +0)) = (beta, -0): a negative entry has
+ * log = log|beta| - i pi in the reference. */
 static void zlq(int d, int n, zc *a, zc *l) {
   zc *at = xcalloc((size_t)n * d, sizeof(zc));
   zc *tau = xcalloc((size_t)d, sizeof(zc));
@@ -1324,7 +1324,7 @@ static void zlq(int d, int n, zc *a, zc *l) {
   zgeqr2(n, d, at, n, tau);
   for (int j = 0; j < d; ++j)
     for (int i = 0; i < d; ++i)
-      l[i + (size_t)d * j] = (i > j) ? conj(at[j + (size_t)n * i]) : (i == j) ? CMPLX(creal(at[j + (size_t)n * j]), 0.0) : 0;
+      l[i + (size_t)d * j] = (i >= j) ? conj(at[j + (size_t)n * i]) : 0;
   zung2r(n, d, at, n, tau);
   conj_transpose(n, d, at, a);
   free(tau);

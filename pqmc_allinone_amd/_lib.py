@@ -23,7 +23,7 @@ class PQMCError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("L1", C.c_int), ("L2", C.c_int), ("Lt", C.c_int), ("ngroup", C.c_int), ("ne", C.c_int),
-                ("block_bonds", C.c_int), ("helper_bonds", C.c_int)]
+                ("block_bonds", C.c_int), ("helper_bonds", C.c_int), ("finite_temperature", C.c_int)]
 
 
 class ModelTables(C.Structure):

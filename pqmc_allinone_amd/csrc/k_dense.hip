@@ -424,7 +424,9 @@ extern "C" hipError_t pqmc_eye(zd *Q, int ldq, int m, int n, hipStream_t s) {
 }
 
 // D = diag(R) and the unit upper R' = D^-1 R of qdr (src/mod_matrixlib.f90:323-334)
-// from the factored A (R in its upper triangle).  conjt stores R'^H (ldq).
+// from the factored A (R in its upper triangle).  conjt stores R'^H (ldq) and
+// D = conj(diag(R)): the factors of ldq (:349-360) of A^H, whose diagonal
+// ZGELQF leaves conjugated, (beta, -0).
 __global__ void k_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int conjt) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
     const int i = e % n, j = e / n;
@@ -432,7 +434,7 @@ __global__ void k_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int c
     zd v = (i <= j) ? zdiv(A[i + (size_t)lda * j], d) : zmk(0.0, 0.0);
     if (conjt) Rout[j + (size_t)n * i] = zconj(v);
     else Rout[i + (size_t)n * j] = v;
-    if (i == j) D[i] = d;
+    if (i == j) D[i] = conjt ? zconj(d) : d;
   }
 }
 
@@ -564,12 +566,12 @@ extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ld
 }
 
 // sum_i log(D(i)) of one D_string block (principal complex logs), into *out
-__global__ void k_dlog(const zd *D, int n, zd *out) {
+__global__ void k_dlog(const zd *D, int n, int conj, zd *out) {
   __shared__ double sr[4], si[4];
   const int tid = threadIdx.x;
   double ar = 0.0, ai = 0.0;
   for (int j = tid; j < n; j += 256) {
-    const zd l = zlog(D[j]);
+    const zd l = zlog(conj ? zconj(D[j]) : D[j]);
     ar += l.x;
     ai += l.y;
   }
@@ -584,7 +586,14 @@ __global__ void k_dlog(const zd *D, int n, zd *out) {
 }
 
 extern "C" hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_dlog, dim3(1), dim3(256), 0, s, D, n, out);
+  hipLaunchKernelGGL(k_dlog, dim3(1), dim3(256), 0, s, D, n, 0, out);
+  return hipGetLastError();
+}
+
+// sum_i log conj(D_i): the diagonal of an LQ (ZGELQF) taken from the QR of
+// the conjugate transpose; a negative real entry then adds -i pi
+extern "C" hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dlog, dim3(1), dim3(256), 0, s, D, n, 1, out);
   return hipGetLastError();
 }
 
@@ -637,6 +646,129 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
 extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                                 int tr_weight, int ncopy, zd *out, hipStream_t s) {
   hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, dlog, nblk, tr_weight, ncopy, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Finite temperature (proj = .false.): the scale-split Green's function of
+// cal_g_and_det_FT (src/mod_evolution.f90:494-648).  A scale d splits into a
+// big part db = (|d| > 1 ? d : 1) and a small part ds = (|d| > 1 ? 1 : d).
+
+PQ_DEV zd ft_big(zd d) { return (d.x * d.x + d.y * d.y > 1.0) ? d : zmk(1.0, 0.0); }
+PQ_DEV zd ft_small(zd d) { return (d.x * d.x + d.y * d.y > 1.0) ? zmk(1.0, 0.0) : d; }
+
+// X (m x n, ld): mode 0 multiplies column j by D[j], mode 2 by conj(D[j]);
+// mode 1 divides row i by the big part of D[i]
+__global__ void k_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  zd &x = X[i + (size_t)ld * j];
+  x = mode == 1 ? zdiv(x, ft_big(D[i])) : zmul(x, mode == 2 ? zconj(D[j]) : D[j]);
+}
+
+// M = db_r^-1 QQ db_l^-1 + ds_r TT ds_l  (n x n, ld n)
+__global__ void k_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const zd *Dl, int n) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e % n), j = (int)(e / n);
+  const zd a = zdiv(zdiv(QQ[e], ft_big(Dr[i])), ft_big(Dl[j]));
+  const zd b = zmul(zmul(ft_small(Dr[i]), TT[e]), ft_small(Dl[j]));
+  M[e] = zadd(a, b);
+}
+
+// out[0] = sum log db_r, out[1] = sum log db_l (principal logs), out[2] =
+// *Lr, out[3] = *Ll (log det of the two unitary factors)
+__global__ void k_ft_dlog(const zd *Dr, const zd *Dl, const zd *Lr, const zd *Ll, int n, zd *out) {
+  __shared__ double s[4][4];
+  const int tid = threadIdx.x;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = tid; j < n; j += 256) {
+    const zd lr = zlog(ft_big(Dr[j])), ll = zlog(ft_big(Dl[j]));
+    a[0] += lr.x;
+    a[1] += lr.y;
+    a[2] += ll.x;
+    a[3] += ll.y;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double v = wave_sum(a[k]);
+    if ((tid & 63) == 0) s[tid >> 6][k] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    out[0] = zmk(s[0][0] + s[1][0] + s[2][0] + s[3][0], s[0][1] + s[1][1] + s[2][1] + s[3][1]);
+    out[1] = zmk(s[0][2] + s[1][2] + s[2][2] + s[3][2], s[0][3] + s[1][3] + s[2][3] + s[3][3]);
+    out[2] = *Lr;
+    out[3] = *Ll;
+  }
+}
+
+// *out = log det Q of a Householder QR (det Q = prod_i (1 - tau_i |v_i|^2)),
+// principal branch, or its conjugate (the determinant of Q^H)
+__global__ void k_ft_qlog(const zd *tau, const double *vn2, int n, int conj, zd *out) {
+  __shared__ double sr[4], si[4];
+  const int tid = threadIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int j = tid; j < n; j += 256) {
+    const zd l = zlog(zsub(zmk(1.0, 0.0), zscale(tau[j], vn2[j])));
+    ar += l.x;
+    ai += l.y;
+  }
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if ((tid & 63) == 0) {
+    sr[tid >> 6] = ar;
+    si[tid >> 6] = ai;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double raw = si[0] + si[1] + si[2] + si[3];
+    const double im = atan2(sin(raw), cos(raw));
+    *out = zmk(sr[0] + sr[1] + sr[2] + sr[3], conj ? -im : im);
+  }
+}
+
+// D[0..n) = 1
+__global__ void k_ft_ones(zd *D, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) D[e] = zmk(1.0, 0.0);
+}
+
+// G <- 1 - G (g_h from (1 + B)^-1)
+__global__ void k_ft_finish(zd *G, int n) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e % n), j = (int)(e / n);
+  G[e] = zsub(zmk(i == j ? 1.0 : 0.0, 0.0), G[e]);
+}
+
+static inline int ft_grid(long n) { return (int)((n + 255) / 256); }
+
+extern "C" hipError_t pqmc_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_scale, dim3(ft_grid((long)m * n)), dim3(256), 0, s, X, ld, m, n, D, mode);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const zd *Dl, int n,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_mix, dim3(ft_grid((long)n * n)), dim3(256), 0, s, M, QQ, TT, Dr, Dl, n);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_dlog(const zd *Dr, const zd *Dl, const zd *Lr, const zd *Ll, int n, zd *out,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_dlog, dim3(1), dim3(256), 0, s, Dr, Dl, Lr, Ll, n, out);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_qlog(const zd *tau, const double *vn2, int n, int conj, zd *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_qlog, dim3(1), dim3(256), 0, s, tau, vn2, n, conj, out);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_ones(zd *D, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_ones, dim3(ft_grid(n)), dim3(256), 0, s, D, n);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_finish(zd *G, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_finish, dim3(ft_grid((long)n * n)), dim3(256), 0, s, G, n);
   return hipGetLastError();
 }
 

@@ -51,11 +51,18 @@ hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncol
 hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                      int tr_weight, int ncopy, zd *out, hipStream_t s);
 hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
+hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
 hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
 hipError_t pqmc_mfma_probe(const double *A, const double *B, double *C, hipStream_t s);
 hipError_t pqmc_launch_measure(const model_t *md, const zd *G, const double *x, int time, const zd *phase,
                                zd *out, hipStream_t s);
+hipError_t pqmc_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode, hipStream_t s);
+hipError_t pqmc_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const zd *Dl, int n, hipStream_t s);
+hipError_t pqmc_ft_dlog(const zd *Dr, const zd *Dl, const zd *Lr, const zd *Ll, int n, zd *out, hipStream_t s);
+hipError_t pqmc_ft_qlog(const zd *tau, const double *vn2, int n, int conj, zd *out, hipStream_t s);
+hipError_t pqmc_ft_ones(zd *D, int n, hipStream_t s);
+hipError_t pqmc_ft_finish(zd *G, int n, hipStream_t s);
 }
 
 static thread_local std::string g_err;
@@ -122,6 +129,14 @@ struct pqmc_handle {
   zd *d_glog = nullptr;  // [nblock + 4] log terms of the weight
   zd *d_gw = nullptr;    // the weight
   double err_fast_max = 1e-6;
+  // finite temperature: per block b an Ns x Ns string B = Q D T (right
+  // blocks, FQ = Q) or T D Q (left blocks, FQ = Q^H), the scales D, the
+  // triangular factor T and log det of the unitary factor
+  int ft = 0;
+  zd *d_FQ = nullptr, *d_FT = nullptr, *d_FD = nullptr, *d_FL = nullptr;
+  zd *d_fw = nullptr;    // five Ns x Ns work matrices
+  zd *d_flog = nullptr;  // [4] log terms of ln det(1 + B)
+  qr_ws qf;              // QR workspace for Ns x Ns
   // timing
   bool timing[3] = {false, false, false};
   ev_pool pools[3];
@@ -196,6 +211,8 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   if (h->ne > h->Ns) { delete h; return fail(PQMC_EINVAL, "ne > Ns"); }
   if (h->Ns > 1024) { delete h; return fail(PQMC_EINVAL, "Ns > 1024 not supported"); }
   if (cfg->block_bonds < 0 || cfg->block_bonds > PS_NB) { delete h; return fail(PQMC_EINVAL, "block_bonds out of range"); }
+  h->ft = cfg->finite_temperature ? 1 : 0;
+  if (h->ft && cfg->Lt % cfg->ngroup) { delete h; return fail(PQMC_EINVAL, "finite temperature needs Lt % ngroup == 0"); }
   h->nb = cfg->block_bonds > 0 ? cfg->block_bonds : PS_NB;
   h->nblk = (h->npla + h->nb - 1) / h->nb;
   {
@@ -239,7 +256,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_D, (size_t)ne * h->nblock);
   A_(h->d_R, (size_t)ne * ne * h->nblock);
   A_(h->d_dlog, (size_t)h->nblock);
-  A_(h->d_rinv, (size_t)256 * ((ne + 15) / 16));
+  A_(h->d_rinv, (size_t)256 * (((h->ft ? Ns : ne) + 15) / 16));
   A_(h->d_slQ, (size_t)Ns * ne);
   A_(h->d_slD, ne);
   A_(h->d_slR, (size_t)ne * ne);
@@ -269,6 +286,20 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_err, 1);
   A_(h->d_meas, h->total_width);
   A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
+  if (h->ft) {
+    const size_t nn = (size_t)Ns * Ns;
+    A_(h->d_FQ, nn * h->nblock);
+    A_(h->d_FT, nn * h->nblock);
+    A_(h->d_FD, (size_t)Ns * h->nblock);
+    A_(h->d_FL, h->nblock);
+    A_(h->d_fw, 5 * nn);
+    A_(h->d_flog, 4);
+    A_(h->qf.V, nn);
+    A_(h->qf.T, (size_t)16 * 16 * ((Ns + 7) / 8));
+    A_(h->qf.tau, Ns);
+    A_(h->qf.beta, Ns);
+    A_(h->qf.vn2, Ns);
+  }
 #undef A_
   if (hipMemsetAsync(h->d_dlog, 0, h->nblock * sizeof(zd), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_psync, 0, pqmc_pass_sync_words(h->nblk) * sizeof(unsigned long long), h->st) != hipSuccess ||
@@ -310,7 +341,8 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
-                  h->d_gx, h->d_gekv, h->d_glog, h->d_gw};
+                  h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
+                  h->d_flog, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -441,11 +473,12 @@ static zd *QS(pqmc_handle *h, int block) { return h->d_Q + (size_t)h->Ns * h->ne
 static zd *DS(pqmc_handle *h, int block) { return h->d_D + (size_t)h->ne * (block - 1); }
 static zd *RS(pqmc_handle *h, int block) { return h->d_R + (size_t)h->ne * h->ne * (block - 1); }
 
-static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backward, const ekv_t *ekv = nullptr) {
+static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backward, const ekv_t *ekv = nullptr,
+                     int ncols = 0) {
   prop_args a;
   a.md = h->md;
   a.X = X;
-  a.ncols = h->ne;
+  a.ncols = ncols > 0 ? ncols : h->ne;
   a.sites = h->d_sites;
   a.ekv = ekv ? ekv : h->d_ekv;
   a.t_first = t_first;
@@ -484,12 +517,137 @@ static int wrap_g(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
   return PQMC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Finite temperature (init_g_FT / get_g_scratch_FT / cal_g_and_det_FT,
+// src/mod_evolution.f90:411-785).  The reference's branch is marked as not in
+// use and is not self-consistent (its init forms (1 + B B^H)^-1 where the
+// commented call beside it forms (1 + B)^-1, and its left string keeps the
+// unitary and triangular factors in swapped roles); this computes what the
+// branch is built for, with its structure: G = (1 + B(beta,0))^-1 from a
+// right string B(t,0) = Q_r D_r T_r and a left string B(beta,t) = T_l D_l Q_l,
+// each advanced by one column- (row-) scaled QR (LQ) per ngroup slices, the
+// scales split at |d| = 1; g_h = 1 - G and ln_cw = ln det(1 + B(beta,0)).
+// A right block keeps Q_r, a left block keeps Q_l^H, so that both strings are
+// advanced by the same one-way propagation and column-pivot-free QR as the
+// projector strings.
+
+static zd *FQS(pqmc_handle *h, int b) { return h->d_FQ + (size_t)h->Ns * h->Ns * (b - 1); }
+static zd *FTS(pqmc_handle *h, int b) { return h->d_FT + (size_t)h->Ns * h->Ns * (b - 1); }
+static zd *FDS(pqmc_handle *h, int b) { return h->d_FD + (size_t)h->Ns * (b - 1); }
+static zd *FW(pqmc_handle *h, int k) { return h->d_fw + (size_t)h->Ns * h->Ns * k; }
+
+static int ft_identity(pqmc_handle *h, int b) {
+  const int Ns = h->Ns;
+  HIPCHK(pqmc_eye(FQS(h, b), Ns, Ns, Ns, h->st));
+  HIPCHK(pqmc_eye(FTS(h, b), Ns, Ns, Ns, h->st));
+  HIPCHK(pqmc_ft_ones(FDS(h, b), Ns, h->st));
+  HIPCHK(hipMemsetAsync(h->d_FL + (b - 1), 0, sizeof(zd), h->st));
+  return PQMC_OK;
+}
+
+// right string of block b from block b-1 and slices t0..t1:
+// B Q_{b-1} D_{b-1} = Q_b D_b T'; T_b = T' T_{b-1}
+static int ft_right(pqmc_handle *h, int b, int t0, int t1) {
+  const int Ns = h->Ns;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  zd *X = FW(h, 0), *Tp = FW(h, 1);
+  HIPCHK(hipMemcpyAsync(X, FQS(h, b - 1), sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
+  int rc = propagate(h, X, t0, t1 - t0 + 1, 0, nullptr, Ns);
+  if (rc) return rc;
+  HIPCHK(pqmc_ft_scale(X, Ns, Ns, Ns, FDS(h, b - 1), 0, h->st));
+  rc = run_qr(h, h->qf, X, Ns, Ns, Ns, FQS(h, b), FDS(h, b), Tp, 0);
+  if (rc) return rc;
+  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 0, h->d_FL + (b - 1), h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tp, Ns, FTS(h, b - 1), Ns, zero, FTS(h, b), Ns, h->st));
+  return PQMC_OK;
+}
+
+// left string of block b from block b+1 and slices t1 down to t0, kept as
+// conjugate transposes: (D_{b+1} Q_{b+1} B)^H = Q_b^H D_b L'^H; T_b = T_{b+1} L'
+static int ft_left(pqmc_handle *h, int b, int t0, int t1) {
+  const int Ns = h->Ns;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  zd *W = FW(h, 0), *Lp = FW(h, 1);
+  HIPCHK(hipMemcpyAsync(W, FQS(h, b + 1), sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
+  int rc = propagate(h, W, t1, t1 - t0 + 1, 1, nullptr, Ns);
+  if (rc) return rc;
+  HIPCHK(pqmc_ft_scale(W, Ns, Ns, Ns, FDS(h, b + 1), 2, h->st));
+  rc = run_qr(h, h->qf, W, Ns, Ns, Ns, FQS(h, b), FDS(h, b), Lp, 1);
+  if (rc) return rc;
+  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 1, h->d_FL + (b - 1), h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, FTS(h, b + 1), Ns, Lp, Ns, zero, FTS(h, b), Ns, h->st));
+  return PQMC_OK;
+}
+
+// G = (1 + Q_r D_r T_r T_l D_l Q_l)^-1 = Q_l^H Db_l^-1 M^-1 Db_r^-1 Q_r^H with
+// M = Db_r^-1 Q_r^H Q_l^H Db_l^-1 + Ds_r T_r T_l Ds_l; Gout = 1 - G.
+// ln det(1 + B) = ln det M + ln det Q_r + ln det Q_l + sum ln Db_r + sum ln Db_l.
+static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout) {
+  const int Ns = h->Ns;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  zd *QQ = FW(h, 0), *TT = FW(h, 1), *M = FW(h, 2), *QM = FW(h, 3), *Z = FW(h, 4);
+  const zd *Qr = FQS(h, br), *Ul = FQS(h, bl), *Dr = FDS(h, br), *Dl = FDS(h, bl);
+  HIPCHK(pqmc_zgemm(1, 0, Ns, Ns, Ns, one, Qr, Ns, Ul, Ns, zero, QQ, Ns, h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, FTS(h, br), Ns, FTS(h, bl), Ns, zero, TT, Ns, h->st));
+  HIPCHK(pqmc_ft_mix(M, QQ, TT, Dr, Dl, Ns, h->st));
+  HIPCHK(pqmc_ft_dlog(Dr, Dl, h->d_FL + (br - 1), h->d_FL + (bl - 1), Ns, h->d_flog, h->st));
+  int rc = run_qr(h, h->qf, M, Ns, Ns, Ns, QM, nullptr, nullptr, 0);
+  if (rc) return rc;
+  // TT <- Db_r^-1 Q_r^H;  Z = R_M^-1 Q_M^H TT;  Z <- Db_l^-1 Z;  G = Q_l^H Z
+  HIPCHK(pqmc_conjt(Qr, Ns, Ns, Ns, TT, Ns, h->st));
+  HIPCHK(pqmc_ft_scale(TT, Ns, Ns, Ns, Dr, 1, h->st));
+  HIPCHK(pqmc_zgemm(1, 0, Ns, Ns, Ns, one, QM, Ns, TT, Ns, zero, Z, Ns, h->st));
+  HIPCHK(pqmc_trsm_upper(M, Ns, Ns, Z, Ns, Ns, h->d_rinv, h->st));
+  HIPCHK(pqmc_ft_scale(Z, Ns, Ns, Ns, Dl, 1, h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Ul, Ns, Z, Ns, zero, Gout, Ns, h->st));
+  HIPCHK(pqmc_ft_finish(Gout, Ns, h->st));
+  HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, h->d_lncw,
+                   h->st));
+  HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
+  return PQMC_OK;
+}
+
+// init_g_FT: the identity at blocks 1 and nblock, right strings of blocks
+// 2..nblock-1, G at the end of the slices
+static int ft_init(pqmc_handle *h) {
+  const int ng = h->ngroup, nb = h->nblock;
+  int rc = ft_identity(h, 1);
+  if (!rc) rc = ft_identity(h, nb);
+  for (int b = 2; b <= nb - 1 && !rc; ++b) rc = ft_right(h, b, (b - 2) * ng + 1, (b - 1) * ng);
+  if (!rc) rc = ft_cal_g(h, nb - 1, nb, h->d_G);
+  return rc;
+}
+
+// get_g_scratch_FT (src/mod_evolution.f90:651-740): the new G into d_Gn
+static int ft_scratch(pqmc_handle *h, int time, int forward) {
+  const int ng = h->ngroup;
+  if (forward) {
+    const int block = time / ng + 1;
+    if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
+    int rc = ft_right(h, block, time - ng, time - 1);
+    return rc ? rc : ft_cal_g(h, block, block + 1, h->d_Gn);
+  }
+  int block = time / ng + 2;
+  if (ng == 1) block -= 1;
+  if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
+  int rc = ft_left(h, block, time, time + ng - 1);
+  return rc ? rc : ft_cal_g(h, block - 1, block, h->d_Gn);
+}
+
 extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
-  if (!h->model_set || !h->trial_set || !h->fields_set) return fail(PQMC_EINVAL, "model, trial and fields must be set");
+  if (!h->model_set || !h->fields_set || (!h->trial_set && !h->ft))
+    return fail(PQMC_EINVAL, h->ft ? "model and fields must be set" : "model, trial and fields must be set");
   if (start_time < 1 || start_time > h->Lt) return fail(PQMC_EINVAL, "start_time out of range");
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, nb = h->nblock;
   HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_x, h->d_bfi, h->d_cross, h->d_ph, h->d_ekv, 1, h->Lt, h->st));
+  if (h->ft) {
+    int rc = ft_init(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->g_ready = true;
+    return PQMC_OK;
+  }
   const size_t qb = sizeof(zd) * Ns * ne;
   HIPCHK(hipMemcpyAsync(QS(h, 1), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
   HIPCHK(hipMemcpyAsync(QS(h, nb), h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
@@ -654,8 +812,11 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
   int rc = PQMC_OK;
   span_guard sg(h, 2);
   const size_t qb = sizeof(zd) * Ns * ne;
-  const zd *lmatH, *rmat;
-  if (forward) {
+  const zd *lmatH = nullptr, *rmat = nullptr;
+  if (h->ft) {
+    rc = ft_scratch(h, time, forward);
+    if (rc) return rc;
+  } else if (forward) {
     const int block = time / ng + 1;
     if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
     HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block - 1), qb, hipMemcpyDeviceToDevice, h->st));
@@ -679,8 +840,10 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     lmatH = QS(h, block);
     rmat = QS(h, block - 1);
   }
-  rc = cal_g(h, lmatH, rmat, h->d_Gn);
-  if (rc) return rc;
+  if (!h->ft) {
+    rc = cal_g(h, lmatH, rmat, h->d_Gn);
+    if (rc) return rc;
+  }
   HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(unsigned long long), h->st));
   HIPCHK(pqmc_maxdiff(h->d_G, h->d_Gn, (long)Ns * Ns, h->d_err, h->st));
   std::swap(h->d_G, h->d_Gn);
@@ -833,6 +996,7 @@ extern "C" int pqmc_set_g(pqmc_handle *h, const double *g) {
 
 extern "C" int pqmc_get_qstring(pqmc_handle *h, int block, double *q) {
   if (!h || !q) return fail(PQMC_EINVAL, "null argument");
+  if (h->ft) return fail(PQMC_EINVAL, "no projector B-string at finite temperature");
   if (block < 1 || block > h->nblock) return fail(PQMC_EINVAL, "block out of range");
   HIPCHK(hipMemcpyAsync(q, QS(h, block), sizeof(zd) * h->Ns * h->ne, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -841,6 +1005,7 @@ extern "C" int pqmc_get_qstring(pqmc_handle *h, int block, double *q) {
 
 extern "C" int pqmc_get_dstring(pqmc_handle *h, double *d) {
   if (!h || !d) return fail(PQMC_EINVAL, "null argument");
+  if (h->ft) return fail(PQMC_EINVAL, "no projector B-string at finite temperature");
   HIPCHK(hipMemcpyAsync(d, h->d_D, sizeof(zd) * h->ne * h->nblock, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   return PQMC_OK;
@@ -865,6 +1030,7 @@ extern "C" int pqmc_get_lncw(pqmc_handle *h, double *ri) {
 // the trial's log D at both ends.  The chain's own state is not touched.
 extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
   if (!h || !x || !ri) return fail(PQMC_EINVAL, "null argument");
+  if (h->ft) return fail(PQMC_EINVAL, "global moves are not available at finite temperature");
   if (!h->model_set || !h->trial_set) return fail(PQMC_EINVAL, "model and trial must be set");
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, Lt = h->Lt;
   const size_t nx = (size_t)h->Nb * Lt;
@@ -890,7 +1056,7 @@ extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
     if (n < ng) break;  // a partial last group has no LQ
     rc = run_qr(h, h->qa, W, Ns, Ns, ne, Qn, nullptr, nullptr, 0);
     if (rc) return rc;
-    HIPCHK(pqmc_dlog(h->qa.beta, ne, h->d_glog + nlog++, h->st));
+    HIPCHK(pqmc_dlog_conj(h->qa.beta, ne, h->d_glog + nlog++, h->st));
     std::swap(W, Qn);
   }
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);

@@ -12,6 +12,7 @@ module pqmc_hip
   type, bind(C) :: pqmc_config
     integer(c_int) :: L1, L2, Lt, ngroup, ne, block_bonds
     integer(c_int) :: helper_bonds = 0
+    integer(c_int) :: finite_temperature = 0
   end type pqmc_config
 
   type, bind(C) :: pqmc_model

@@ -147,11 +147,16 @@ def propose_global(model: Model, x: np.ndarray, kind: str, rng: "Ran2", distance
 class Chain:
     """One projector-DQMC Markov chain resident on the GPU."""
 
-    def __init__(self, model: Model, block_bonds: int = 0, helper_bonds: int = 0):
+    def __init__(self, model: Model, block_bonds: int = 0, helper_bonds: int = 0,
+                 finite_temperature: bool = False):
+        """finite_temperature: G = (1 + B(beta,0))^-1 (proj = .false. of
+        src/input.f90) instead of the projector G; needs no trial matrix."""
         self.model = model
         p = model.p
         self._L = _lib.lib()
-        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(block_bonds), int(helper_bonds))
+        self.finite_temperature = bool(finite_temperature)
+        cfg = _lib.Config(p.L1, p.L2, p.Lt, p.ngroup, p.ne, int(block_bonds), int(helper_bonds),
+                          int(self.finite_temperature))
         h = C.c_void_p()
         _lib.check(self._L.pqmc_create(C.byref(cfg), C.byref(h)))
         self.h = h

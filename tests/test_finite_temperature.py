@@ -87,3 +87,90 @@ def test_ft_midsweep_stabilization_matches_dense():
     t = Lt - 2 * ng + 1
     ref, tol = g_ft(m, xf, t - 1)
     assert np.abs(orc.g - ref).max() < max(tol, 1e-9)
+
+
+# ---- the device path against the oracle -------------------------------------
+
+SP_TWO_PI = float(np.float32(2 * np.pi))
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def ft_chain(m, x):
+    from pqmc_allinone_amd.host import Chain
+    ch = Chain(m, finite_temperature=True)
+    ch.set_fields(x)
+    return ch
+
+
+def same_lncw(a, b):
+    """Re to 1e-9 relative; Im to 1e-8 up to one 2 pi_SP of the final
+    reduction (the branches of every log term agree, so the unreduced sums
+    agree)."""
+    if abs(a.real - b.real) > 1e-9 * max(1.0, abs(b.real)):
+        return False
+    d = abs(a.imag - b.imag)
+    return min(d, abs(d - SP_TWO_PI)) < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,Lt,ng", [(2, 12, 3), (4, 20, 5), (4, 40, 10)])
+def test_gpu_ft_init_matches_oracle(L, Lt, ng):
+    m, x, orc = ft_pair(L, Lt, ng)
+    orc.init(1)
+    ch = ft_chain(m, x)
+    print(f"ft init L={L} Lt={Lt} ng={ng}", flush=True)
+    ch.init_g(1)
+    e = rel(ch.g_h, orc.g)
+    print(f"  rel G err {e:.3e}  lncw gpu {ch.ln_cw} oracle {orc.ln_cw}", flush=True)
+    assert e < 1e-10
+    assert same_lncw(ch.ln_cw, orc.ln_cw)
+    ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,Lt,ng", [(2, 30, 5), (4, 20, 5), (6, 40, 10)])
+def test_gpu_ft_sweeps_match_oracle(L, Lt, ng):
+    """Four sweeps, alternating direction, with measurements: identical
+    accepts and fields, G, ln_cw and measurements to 1e-9."""
+    m, x, orc = ft_pair(L, Lt, ng)
+    orc.init(1)
+    ch = ft_chain(m, x)
+    ch.init_g(1)
+    rng = O.Ran2Stream(17)
+    fw, tot = True, 0
+    for sweep in range(4):
+        fw = not fw
+        u = rng.draws(2 * m.Nb * Lt)
+        print(f"ft sweep L={L} Lt={Lt} ng={ng} sweep={sweep} forward={fw}", flush=True)
+        mg = ch.sweep(fw, u, measure=True)
+        acc_o, mo = orc.sweep(fw, u, measure=True)
+        acc = ch.accepts()
+        assert np.array_equal(acc, acc_o), sweep
+        assert np.array_equal(ch.get_fields(), orc.get_fields()), sweep
+        e = rel(ch.g_h, orc.g)
+        print(f"  accepts {int(acc.sum())} rel G err {e:.3e} meas err {rel(mg, mo):.3e}", flush=True)
+        assert e < 1e-9, sweep
+        assert rel(mg, mo) < 1e-9, sweep
+        assert same_lncw(ch.ln_cw, orc.ln_cw), (sweep, ch.ln_cw, orc.ln_cw)
+        tot += int(acc.sum())
+    assert tot > 0
+    ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_ft_refusals():
+    """Lt not a multiple of ngroup, global weights and the projector string
+    readers are refused for a finite-temperature chain."""
+    from pqmc_allinone_amd._lib import PQMCError
+    from pqmc_allinone_amd.host import Chain
+    with pytest.raises(PQMCError, match="Lt % ngroup"):
+        Chain(build_model(ModelParams(L1=2, L2=2, Lt=12, ngroup=5, ne=4)), finite_temperature=True)
+    m, x, _ = ft_pair(2, 12, 3)
+    ch = ft_chain(m, x)
+    ch.init_g(1)
+    with pytest.raises(PQMCError, match="finite temperature"):
+        ch.global_weight(x)
+    ch.close()

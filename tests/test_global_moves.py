@@ -66,11 +66,12 @@ def test_oracle_new_weight_equals_fresh_init_weight():
 
 
 def test_oracle_new_weight_log_branches_follow_lapack():
-    """Im ln_cw term by term on the reference's branches, with numpy's LAPACK
-    (zgeqrf) for the LQ of src/mod_matrixlib.f90:488-503: its diagonal is
-    (beta, +0), so a negative beta adds +i pi, and Im is reduced mod 2*pi in
-    REAL(SP) only at the end.  A -i pi branch would leave a residue of
-    (2 pi - 2 pi_SP) per negative beta after that reduction."""
+    """Im ln_cw term by term on the reference's branches: the LQ of
+    src/mod_matrixlib.f90:488-503 (ZGELQF) leaves its diagonal as
+    (beta, -0) (pinned against LAPACK in test_oracle_physics), so a negative
+    beta adds -i pi, and Im is reduced mod 2*pi in REAL(SP) only at the end.
+    A +i pi branch would leave a residue of (2 pi - 2 pi_SP) per negative
+    beta after that reduction."""
     L, Lt, ng = 4, 36, 5
     m, x, sl, orc = oracle_for(L, Lt, ng, L * L)
     q, d, _ = O.zqdr(sl)
@@ -83,7 +84,7 @@ def test_oracle_new_weight_log_branches_follow_lapack():
             qq, r = np.linalg.qr(lmat.conj().T)
             beta = np.diag(r).real
             nneg += int((beta < 0).sum())
-            lnd += np.log(beta + 0j).sum()
+            lnd += np.log(np.conj(beta + 0j)).sum()
             lmat = qq.conj().T
     lnd += np.log(d).sum() + np.log(np.linalg.det(lmat @ q))
     w = orc.cal_new_weight()

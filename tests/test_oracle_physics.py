@@ -310,3 +310,34 @@ def test_boson_correlators_match_numpy():
         assert abs(out[lay["BFx_BFy"] + d] - (bx * sh(by)).mean()) < 1e-12
         assert abs(out[lay["BFxy_BFxy"] + d] - (bx * by * sh(bx) * sh(by)).mean()) < 1e-12
     assert abs(out[lay["BF_X"]] - x[t - 1].sum() / m.Ncell) < 1e-12
+
+
+def test_factorizations_follow_lapack():
+    """The oracle's qdr and ldq (src/mod_matrixlib.f90:323-360) against
+    LAPACK's ZGEQRF and ZGELQF: the same triangular factors, and the same
+    sign of zero in the imaginary part of the diagonal, (beta, +0) for QR and
+    (beta, -0) for LQ, which fixes the branch of log(d) for negative d."""
+    import lapack_ref
+    if not lapack_ref.available():
+        pytest.skip("no LAPACK with zgeqrf/zgelqf found")
+    rng = np.random.default_rng(11)
+    nneg = 0
+    for m, n in [(7, 4), (9, 9), (12, 5)]:
+        a = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        fr, _ = lapack_ref.zgeqrf(a)
+        q, d, r = O.zqdr(a)
+        dr = np.diag(fr)[:n]
+        assert np.allclose(d, dr, rtol=1e-13, atol=0)
+        assert np.array_equal(np.signbit(d.imag), np.signbit(dr.imag))
+        assert np.allclose(np.triu(fr[:n, :n]) / dr[:, None], r, rtol=0, atol=1e-13)
+        b = a.T.copy()  # n x m, n <= m
+        fl, _ = lapack_ref.zgelqf(b)
+        qq, dl, l = O.zldq(b)
+        dlap = np.diag(fl)[:n]
+        assert np.allclose(dl, dlap, rtol=1e-13, atol=0)
+        assert np.array_equal(np.signbit(dl.imag), np.signbit(dlap.imag))
+        assert np.all(np.signbit(dlap.imag))
+        assert np.allclose(np.tril(fl[:n, :n]) / dlap[None, :], l, rtol=0, atol=1e-13)
+        assert np.allclose(np.log(dl), np.log(dlap), rtol=0, atol=1e-13)
+        nneg += int((dl.real < 0).sum())
+    assert nneg > 0  # some diagonal is negative, so the branch is exercised
