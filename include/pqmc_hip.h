@@ -47,7 +47,7 @@ typedef struct pqmc_config {
   int finite_temperature;  /* 0: projector (proj = .true.); 1: finite temperature, G = (1 + B(beta,0))^-1
                               from Ns x Ns UDT strings (init_g_FT / get_g_scratch_FT,
                               src/mod_evolution.f90:411-785).  Needs Lt % ngroup == 0; no trial
-                              matrix; pqmc_global_weight and the T0 string readers refuse it. */
+                              matrix; the projector string readers refuse it. */
 } pqmc_config;
 
 /* Model tables as built by set_typed_pf / init_pf_para
@@ -148,7 +148,8 @@ int pqmc_synchronize(pqmc_handle *h);
 
 /* Global moves: the full-determinant weight of the field configuration x
  * (Nb*Lt, boson_field layout), replacing cal_new_weight_T0
- * (src/mod_evolution.f90:359-408) as called by update_global
+ * (src/mod_evolution.f90:359-408), or cal_new_weight_FT (:743-785) for a
+ * finite-temperature chain, as called by update_global
  * (src/mod_update.f90:251-290).  The chain's fields, Green's function and
  * B-strings are left unchanged; after an accepted move the caller sets the
  * fields and runs pqmc_init_g(1), as src/Main_PQMC.f90:94 does. */

@@ -1333,7 +1333,38 @@ static void zlq(int d, int n, zc *a, zc *l) {
 
 /* cal_new_weight_T0 (src/mod_evolution.f90:359-408): the full-determinant
  * weight of the current fields, from the trial strings at both ends. */
+/* cal_new_weight_FT (src/mod_evolution.f90:743-785): ln det(1 + B(beta,0))
+ * of the current expKV, from a right string built from the identity over all
+ * slices and the identity left string of block nblock.  The reference scales
+ * the rows of Q by D where its comment says Q' = Q*D (:771-773); this
+ * restates Q*D, the same step as get_g_scratch.  The strings and G are left
+ * as they were. */
+static void ft_new_weight(orc_model *m, double *ri) {
+  size_t nq = (size_t)m->Ns * m->Ns * m->nblock, nd = (size_t)m->Ns * m->nblock, ng = (size_t)m->Ns * m->Ns;
+  zc *q = xcalloc(nq, sizeof(zc)), *t = xcalloc(nq, sizeof(zc)), *d = xcalloc(nd, sizeof(zc));
+  zc *g = xcalloc(ng, sizeof(zc));
+  memcpy(q, m->FQ, nq * sizeof(zc));
+  memcpy(t, m->FT, nq * sizeof(zc));
+  memcpy(d, m->FD, nd * sizeof(zc));
+  memcpy(g, m->g, ng * sizeof(zc));
+  ft_init(m, 1);
+  ri[0] = creal(m->ln_cw);
+  ri[1] = cimag(m->ln_cw);
+  memcpy(m->FQ, q, nq * sizeof(zc));
+  memcpy(m->FT, t, nq * sizeof(zc));
+  memcpy(m->FD, d, nd * sizeof(zc));
+  memcpy(m->g, g, ng * sizeof(zc));
+  free(q);
+  free(t);
+  free(d);
+  free(g);
+}
+
 void orc_cal_new_weight(orc_model *m, double *ri) {
+  if (m->ft) {
+    ft_new_weight(m, ri);
+    return;
+  }
   int ne = m->ne, Ns = m->Ns, ng = m->ngroup;
   zc *lmat = xcalloc((size_t)ne * Ns, sizeof(zc));
   zc *tmat = xcalloc((size_t)ne * ne, sizeof(zc));

@@ -128,6 +128,7 @@ struct pqmc_handle {
   ekv_t *d_gekv = nullptr;
   zd *d_glog = nullptr;  // [nblock + 4] log terms of the weight
   zd *d_gw = nullptr;    // the weight
+  zd *d_gft = nullptr;   // finite temperature: two rolling Ns x Ns UDT strings
   double err_fast_max = 1e-6;
   // finite temperature: per block b an Ns x Ns string B = Q D T (right
   // blocks, FQ = Q) or T D Q (left blocks, FQ = Q^H), the scales D, the
@@ -342,7 +343,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
-                  h->d_flog, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2};
+                  h->d_flog, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -545,21 +546,32 @@ static int ft_identity(pqmc_handle *h, int b) {
   return PQMC_OK;
 }
 
-// right string of block b from block b-1 and slices t0..t1:
-// B Q_{b-1} D_{b-1} = Q_b D_b T'; T_b = T' T_{b-1}
-static int ft_right(pqmc_handle *h, int b, int t0, int t1) {
+// one right-string step over slices t0..t1 with the exponentials ekv:
+// B Q D = Q' D' T'' (column-scaled QR), T' = T'' T; L' = log det Q'
+struct ft_udt {
+  zd *Q, *D, *T, *L;
+};
+
+static int ft_right_step(pqmc_handle *h, ft_udt in, ft_udt out, int t0, int t1, const ekv_t *ekv) {
   const int Ns = h->Ns;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   zd *X = FW(h, 0), *Tp = FW(h, 1);
-  HIPCHK(hipMemcpyAsync(X, FQS(h, b - 1), sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
-  int rc = propagate(h, X, t0, t1 - t0 + 1, 0, nullptr, Ns);
+  HIPCHK(hipMemcpyAsync(X, in.Q, sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
+  int rc = propagate(h, X, t0, t1 - t0 + 1, 0, ekv, Ns);
   if (rc) return rc;
-  HIPCHK(pqmc_ft_scale(X, Ns, Ns, Ns, FDS(h, b - 1), 0, h->st));
-  rc = run_qr(h, h->qf, X, Ns, Ns, Ns, FQS(h, b), FDS(h, b), Tp, 0);
+  HIPCHK(pqmc_ft_scale(X, Ns, Ns, Ns, in.D, 0, h->st));
+  rc = run_qr(h, h->qf, X, Ns, Ns, Ns, out.Q, out.D, Tp, 0);
   if (rc) return rc;
-  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 0, h->d_FL + (b - 1), h->st));
-  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tp, Ns, FTS(h, b - 1), Ns, zero, FTS(h, b), Ns, h->st));
+  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 0, out.L, h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tp, Ns, in.T, Ns, zero, out.T, Ns, h->st));
   return PQMC_OK;
+}
+
+static ft_udt ft_block(pqmc_handle *h, int b) { return {FQS(h, b), FDS(h, b), FTS(h, b), h->d_FL + (b - 1)}; }
+
+// right string of block b from block b-1 and slices t0..t1
+static int ft_right(pqmc_handle *h, int b, int t0, int t1) {
+  return ft_right_step(h, ft_block(h, b - 1), ft_block(h, b), t0, t1, h->d_ekv);
 }
 
 // left string of block b from block b+1 and slices t1 down to t0, kept as
@@ -579,19 +591,32 @@ static int ft_left(pqmc_handle *h, int b, int t0, int t1) {
   return PQMC_OK;
 }
 
-// G = (1 + Q_r D_r T_r T_l D_l Q_l)^-1 = Q_l^H Db_l^-1 M^-1 Db_r^-1 Q_r^H with
-// M = Db_r^-1 Q_r^H Q_l^H Db_l^-1 + Ds_r T_r T_l Ds_l; Gout = 1 - G.
-// ln det(1 + B) = ln det M + ln det Q_r + ln det Q_l + sum ln Db_r + sum ln Db_l.
+// M = Db_r^-1 Q_r^H Q_l^H Db_l^-1 + Ds_r T_r T_l Ds_l in FW(2), factored by
+// a QR (explicit Q into QM unless null), and
+// ln det(1 + B) = ln det M + ln det Q_r + ln det Q_l + sum ln Db_r + sum ln Db_l
+// into *lncw (cal_g_and_det_FT, src/mod_evolution.f90:494-648)
+static int ft_weight(pqmc_handle *h, ft_udt r, ft_udt l, zd *QM, zd *lncw) {
+  const int Ns = h->Ns;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  zd *QQ = FW(h, 0), *TT = FW(h, 1), *M = FW(h, 2);
+  HIPCHK(pqmc_zgemm(1, 0, Ns, Ns, Ns, one, r.Q, Ns, l.Q, Ns, zero, QQ, Ns, h->st));
+  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, r.T, Ns, l.T, Ns, zero, TT, Ns, h->st));
+  HIPCHK(pqmc_ft_mix(M, QQ, TT, r.D, l.D, Ns, h->st));
+  HIPCHK(pqmc_ft_dlog(r.D, l.D, r.L, l.L, Ns, h->d_flog, h->st));
+  int rc = run_qr(h, h->qf, M, Ns, Ns, Ns, QM, nullptr, nullptr, 0);
+  if (rc) return rc;
+  HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, lncw, h->st));
+  return PQMC_OK;
+}
+
+// G = (1 + Q_r D_r T_r T_l D_l Q_l)^-1 = Q_l^H Db_l^-1 M^-1 Db_r^-1 Q_r^H;
+// Gout = 1 - G; ln_cw = ln det(1 + B)
 static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout) {
   const int Ns = h->Ns;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  zd *QQ = FW(h, 0), *TT = FW(h, 1), *M = FW(h, 2), *QM = FW(h, 3), *Z = FW(h, 4);
+  zd *TT = FW(h, 1), *M = FW(h, 2), *QM = FW(h, 3), *Z = FW(h, 4);
   const zd *Qr = FQS(h, br), *Ul = FQS(h, bl), *Dr = FDS(h, br), *Dl = FDS(h, bl);
-  HIPCHK(pqmc_zgemm(1, 0, Ns, Ns, Ns, one, Qr, Ns, Ul, Ns, zero, QQ, Ns, h->st));
-  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, FTS(h, br), Ns, FTS(h, bl), Ns, zero, TT, Ns, h->st));
-  HIPCHK(pqmc_ft_mix(M, QQ, TT, Dr, Dl, Ns, h->st));
-  HIPCHK(pqmc_ft_dlog(Dr, Dl, h->d_FL + (br - 1), h->d_FL + (bl - 1), Ns, h->d_flog, h->st));
-  int rc = run_qr(h, h->qf, M, Ns, Ns, Ns, QM, nullptr, nullptr, 0);
+  int rc = ft_weight(h, ft_block(h, br), ft_block(h, bl), QM, h->d_lncw);
   if (rc) return rc;
   // TT <- Db_r^-1 Q_r^H;  Z = R_M^-1 Q_M^H TT;  Z <- Db_l^-1 Z;  G = Q_l^H Z
   HIPCHK(pqmc_conjt(Qr, Ns, Ns, Ns, TT, Ns, h->st));
@@ -601,8 +626,6 @@ static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout) {
   HIPCHK(pqmc_ft_scale(Z, Ns, Ns, Ns, Dl, 1, h->st));
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Ul, Ns, Z, Ns, zero, Gout, Ns, h->st));
   HIPCHK(pqmc_ft_finish(Gout, Ns, h->st));
-  HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, h->d_lncw,
-                   h->st));
   HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
   return PQMC_OK;
 }
@@ -1023,6 +1046,58 @@ extern "C" int pqmc_get_lncw(pqmc_handle *h, double *ri) {
   return PQMC_OK;
 }
 
+static int global_fields(pqmc_handle *h, const double *x) {
+  const size_t nx = (size_t)h->Nb * h->Lt;
+  if (!h->d_gx) {
+    HIPCHK(dalloc(&h->d_gx, nx));
+    HIPCHK(dalloc(&h->d_gekv, (size_t)h->Lt * 4 * h->npla));
+    HIPCHK(dalloc(&h->d_glog, (size_t)h->nblock + 4));
+    HIPCHK(dalloc(&h->d_gw, 1));
+  }
+  HIPCHK(hipMemcpyAsync(h->d_gx, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->st));
+  HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_gx, h->d_bfi, h->d_cross, h->d_ph, h->d_gekv, 1, h->Lt, h->st));
+  return PQMC_OK;
+}
+
+// cal_new_weight_FT (src/mod_evolution.f90:743-785) for the fields x: a
+// right string from the identity over all slices (two rolling UDT buffers),
+// then ln det(1 + B) against the identity as the left string of block
+// nblock.  The reference scales the rows of Q by D where its comment says
+// Q' = Q*D (:771-773); this takes Q*D, the step get_g_scratch takes.  The
+// chain's strings and G are not touched, and need not exist (update_global
+// may run with fields set by an accepted move, before init_g).
+static int ft_global_weight(pqmc_handle *h, const double *x, double *ri) {
+  if (!h->model_set) return fail(PQMC_EINVAL, "model must be set");
+  const int Ns = h->Ns, ng = h->ngroup, nb = h->nblock;
+  const size_t nn = (size_t)Ns * Ns;
+  int rc = global_fields(h, x);
+  if (rc) return rc;
+  ft_udt buf[3];  // two rolling strings and the identity
+  if (!h->d_gft) HIPCHK(dalloc(&h->d_gft, 6 * nn + 3 * (size_t)Ns + 3));
+  for (int k = 0; k < 3; ++k)
+    buf[k] = {h->d_gft + 2 * k * nn, h->d_gft + 6 * nn + k * Ns, h->d_gft + (2 * k + 1) * nn,
+              h->d_gft + 6 * nn + 3 * Ns + k};
+  const ft_udt id = buf[2];
+  HIPCHK(pqmc_eye(id.Q, Ns, Ns, Ns, h->st));
+  HIPCHK(pqmc_eye(id.T, Ns, Ns, Ns, h->st));
+  HIPCHK(pqmc_ft_ones(id.D, Ns, h->st));
+  HIPCHK(hipMemsetAsync(id.L, 0, sizeof(zd), h->st));
+  ft_udt cur = id;
+  for (int b = 2; b <= nb - 1; ++b) {
+    rc = ft_right_step(h, cur, buf[b & 1], (b - 2) * ng + 1, (b - 1) * ng, h->d_gekv);
+    if (rc) return rc;
+    cur = buf[b & 1];
+  }
+  rc = ft_weight(h, cur, id, nullptr, h->d_gw);
+  if (rc) return rc;
+  zd v;
+  HIPCHK(hipMemcpyAsync(&v, h->d_gw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  ri[0] = v.x;
+  ri[1] = v.y;
+  return PQMC_OK;
+}
+
 // cal_new_weight_T0 (src/mod_evolution.f90:359-408) for the fields x: the
 // left trial string propagated down through all slices with an LQ every
 // ngroup slices (the QR of its conjugate transpose, whose real diagonal is
@@ -1030,24 +1105,17 @@ extern "C" int pqmc_get_lncw(pqmc_handle *h, double *ri) {
 // the trial's log D at both ends.  The chain's own state is not touched.
 extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
   if (!h || !x || !ri) return fail(PQMC_EINVAL, "null argument");
-  if (h->ft) return fail(PQMC_EINVAL, "global moves are not available at finite temperature");
+  if (h->ft) return ft_global_weight(h, x, ri);
   if (!h->model_set || !h->trial_set) return fail(PQMC_EINVAL, "model and trial must be set");
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup, Lt = h->Lt;
-  const size_t nx = (size_t)h->Nb * Lt;
-  if (!h->d_gx) {
-    HIPCHK(dalloc(&h->d_gx, nx));
-    HIPCHK(dalloc(&h->d_gekv, (size_t)Lt * 4 * h->npla));
-    HIPCHK(dalloc(&h->d_glog, (size_t)h->nblock + 4));
-    HIPCHK(dalloc(&h->d_gw, 1));
-  }
-  HIPCHK(hipMemcpyAsync(h->d_gx, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->st));
-  HIPCHK(pqmc_launch_ekv_build(&h->md, h->d_gx, h->d_bfi, h->d_cross, h->d_ph, h->d_gekv, 1, Lt, h->st));
+  int rc = global_fields(h, x);
+  if (rc) return rc;
   const size_t qb = sizeof(zd) * Ns * ne;
   zd *W = h->d_W, *Qn = h->d_scr;  // string (as lmat^H) and the QR's explicit Q
   HIPCHK(hipMemcpyAsync(W, h->d_slQ, qb, hipMemcpyDeviceToDevice, h->st));
   HIPCHK(pqmc_dlog(h->d_slD, ne, h->d_glog, h->st));      // D_string(:, nblock)
   HIPCHK(pqmc_dlog(h->d_slD, ne, h->d_glog + 1, h->st));  // D_string(:, 1)
-  int nlog = 2, rc = PQMC_OK;
+  int nlog = 2;
   for (int p = Lt; p >= 1;) {
     const int n = std::min(ng, p);
     rc = propagate(h, W, p, n, 1, h->d_gekv);

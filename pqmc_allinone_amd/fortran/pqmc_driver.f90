@@ -17,7 +17,8 @@
 ! global update off); float64 global_update_distance (times char_length);
 ! int32 nbins (0 = no bins or files), nmeas_per_bin, meas_number, warmup,
 ! meas_interval, checkpoint loop (save after it, 0 = none), resume (1 =
-! start from checkpoint.bin).
+! start from checkpoint.bin), proj (1 = projector; 0 = finite temperature,
+! src/input.f90 proj; the trial matrix is then read but not used).
 ! Output: int8 accept flags of every update call in call order
 ! (nsweeps * Lt * 4 * npla); complex128 G(Ns, Ns) at the end; float64
 ! fields(Nb, Lt); complex128 summed measurements (total_width); int32 number
@@ -30,7 +31,7 @@ program pqmc_driver
   implicit none
   character(len=1024) :: fin, fout
   integer :: u_in, u_out
-  integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop, mc(7)
+  integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop, mc(8)
   character(len=1024) :: outdir
   integer :: nbins, nmeas, mnum, warmup, meas_interval, ckpt_loop, loop0
   logical :: resume
@@ -70,6 +71,7 @@ program pqmc_driver
   close (u_in)
 
   cfg = pqmc_config(L1, L2, Lt, ng, ne, nbb)
+  if (mc(8) == 0) cfg%finite_temperature = 1
   model%sites = c_loc(sites); model%bfi = c_loc(bfi); model%cross = c_loc(cross); model%bcph = c_loc(bcph)
   model%kcoe = c_loc(kcoe); model%vcoe = c_loc(vcoe); model%k12 = c_loc(k12); model%v12 = c_loc(v12)
   model%bc = c_loc(bc)

@@ -323,10 +323,12 @@ FORTRAN_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortr
 def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.ndarray, nsweeps: int, seed: int,
                        block_bonds: int = 0, global_update_loop: int = 0, global_distance: float = 1.0,
                        nbins: int = 0, nmeas_per_bin: int = 0, meas_number: int = 0, warmup: int = 0,
-                       meas_interval: int = 1, checkpoint_loop: int = 0, resume: bool = False):
+                       meas_interval: int = 1, checkpoint_loop: int = 0, resume: bool = False,
+                       proj: bool = True):
     """Input file of fortran/pqmc_driver (layout in its header comment).
     global_distance is the input's global_update_distance, before its scaling
-    by char_length (src/input.f90:184)."""
+    by char_length (src/input.f90:184).  proj = False runs the finite-
+    temperature Green's function (the input's proj flag)."""
     p = model.p
     with open(path, "wb") as f:
         np.array([p.L1, p.L2, p.Lt, p.ngroup, p.ne, block_bonds, nsweeps, seed], np.int32).tofile(f)
@@ -345,8 +347,8 @@ def write_driver_input(path: str, model: Model, fields: np.ndarray, slater: np.n
         np.asfortranarray(slater, np.complex128).ravel(order="F").tofile(f)
         np.array([global_update_loop], np.int32).tofile(f)
         np.array([global_distance * model.char_length], np.float64).tofile(f)
-        np.array([nbins, nmeas_per_bin, meas_number, warmup, meas_interval, checkpoint_loop, int(bool(resume))],
-                 np.int32).tofile(f)
+        np.array([nbins, nmeas_per_bin, meas_number, warmup, meas_interval, checkpoint_loop, int(bool(resume)),
+                  int(bool(proj))], np.int32).tofile(f)
 
 
 def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:

@@ -2,9 +2,10 @@
 get_g_scratch_FT and cal_g_and_det_FT (src/mod_evolution.f90:411-785).
 
 The oracle is pinned against dense products on small lattices:
-g_h = 1 - (1 + B(tau,0) B(beta,tau))^-1 and Re ln_cw = ln|det(1 + B(beta,0))|,
-after init, at stabilizations in both directions and after full sweeps
-(including accepted updates)."""
+g_h = 1 - (1 + B(tau,0) B(beta,tau))^-1, Re ln_cw = ln|det(1 + B(beta,0))| and
+Im ln_cw = arg det(1 + B(beta,0)) (mod 2 pi), after init, at stabilizations in
+both directions and after full sweeps (including accepted updates).  The
+device path is compared with the oracle (marked gpu)."""
 import numpy as np
 import pytest
 
@@ -48,9 +49,17 @@ def test_ft_init_matches_dense(L, Lt, ng):
     ref, tol = g_ft(m, x, Lt)
     assert tol < 1e-5  # the dense reference still holds five digits
     assert np.abs(orc.g - ref).max() < tol
-    B = dense_b(m, x, Lt)
-    lw = np.linalg.slogdet(np.eye(m.Ns) + B)[1]
-    assert abs(orc.ln_cw.real - lw) < 1e-9 * max(1.0, abs(lw))
+    check_lncw_dense(m, x, orc.ln_cw)
+
+
+def check_lncw_dense(m, x, lncw):
+    """Re ln_cw = ln|det(1 + B(beta,0))|; Im ln_cw = arg det(1 + B) up to
+    2 pi (tr_weight off, ncopy 1).  The final mod 2 pi_SP reduction leaves
+    n (2 pi_SP - 2 pi), n below (Ns + 3) / 2 (one +-pi per log term)."""
+    sign, lw = np.linalg.slogdet(np.eye(m.Ns) + dense_b(m, x, m.p.Lt))
+    assert abs(lncw.real - lw) < 1e-9 * max(1.0, abs(lw))
+    dphi = np.angle(np.exp(1j * lncw.imag) / sign)
+    assert abs(dphi) < 1e-9 + (m.Ns + 3) / 2 * 1.8e-7
 
 
 @pytest.mark.parametrize("L,Lt,ng", [(2, 30, 5), (4, 20, 5)])
@@ -70,6 +79,8 @@ def test_ft_sweeps_match_dense(L, Lt, ng):
         xf = orc.get_fields()
         ref, tol = g_ft(m, xf, 0 if not fw else Lt)
         assert np.abs(orc.g - ref).max() < max(tol, 1e-9)
+        if fw:  # the last stabilization ends the sweep: no accepts after it
+            check_lncw_dense(m, xf, orc.ln_cw)
     assert tot > 0
     pa, na, efm = orc.counters()
     assert efm <= 1e-6
@@ -162,8 +173,8 @@ def test_gpu_ft_sweeps_match_oracle(L, Lt, ng):
 
 @pytest.mark.gpu
 def test_gpu_ft_refusals():
-    """Lt not a multiple of ngroup, global weights and the projector string
-    readers are refused for a finite-temperature chain."""
+    """Lt not a multiple of ngroup and the projector string readers are
+    refused for a finite-temperature chain."""
     from pqmc_allinone_amd._lib import PQMCError
     from pqmc_allinone_amd.host import Chain
     with pytest.raises(PQMCError, match="Lt % ngroup"):
@@ -172,5 +183,112 @@ def test_gpu_ft_refusals():
     ch = ft_chain(m, x)
     ch.init_g(1)
     with pytest.raises(PQMCError, match="finite temperature"):
-        ch.global_weight(x)
+        ch.q_string(1)
+    with pytest.raises(PQMCError, match="finite temperature"):
+        ch.d_string()
+    ch.close()
+
+
+# ---- global moves at finite temperature (cal_new_weight_FT) -----------------
+
+
+def test_ft_new_weight_matches_dense_and_leaves_the_chain():
+    """cal_new_weight_FT of proposed fields is ln det(1 + B(beta,0)) of those
+    fields; the strings and G of the chain are unchanged."""
+    L, Lt, ng = 2, 20, 5
+    m, x, orc = ft_pair(L, Lt, ng)
+    orc.init(1)
+    g0 = orc.g.copy()
+    xn = x + 0.3 * np.random.default_rng(8).normal(size=x.shape)
+    orc.set_fields(xn)
+    orc.init_expkv()
+    w = orc.cal_new_weight()
+    check_lncw_dense(m, xn, w)
+    assert np.array_equal(orc.g, g0)
+    orc.init(1)
+    assert abs(orc.ln_cw.real - w.real) < 1e-12 * abs(w.real)
+
+
+@pytest.mark.parametrize("distance", [1.0, 0.05])
+def test_ft_host_global_moves_match_oracle(distance):
+    """The host's GlobalMoves on an oracle-backed finite-temperature chain
+    against the oracle's update_global: same accepts, fields and ln_cw."""
+    from pqmc_allinone_amd.host import GLOBAL_KINDS, GlobalMoves, Ran2
+    from test_global_moves import KIND_ID, _OracleChain
+    m, x, orc = ft_pair(4, 20, 5, seed=21)
+    _, _, twin = ft_pair(4, 20, 5, seed=21)
+    orc.init(1)
+    twin.init(1)
+    gm = GlobalMoves(_OracleChain(m, twin), Ran2(99), distance)
+    ro = O.Ran2Stream(99)
+    n_acc = 0
+    for _ in range(4):
+        flags_g = gm.run()
+        flags_o = [orc.update_global(KIND_ID[k], ro, distance) for k in GLOBAL_KINDS]
+        if any(flags_o):
+            orc.init(1)
+        assert flags_g == flags_o
+        n_acc += sum(flags_o)
+        assert np.array_equal(twin.get_fields(), orc.get_fields())
+        assert abs(twin.ln_cw - orc.ln_cw) < 1e-9 * abs(orc.ln_cw)
+    if distance < 0.1:
+        assert n_acc > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,Lt,ng", [(2, 30, 5), (4, 20, 5), (6, 40, 10)])
+def test_gpu_ft_global_weight_matches_oracle(L, Lt, ng):
+    from pqmc_allinone_amd.host import GLOBAL_KINDS
+    from test_global_moves import KIND_ID
+    m, x, orc = ft_pair(L, Lt, ng)
+    orc.init(1)
+    ch = ft_chain(m, x)
+    ch.init_g(1)
+    g0 = ch.g_h
+    for kind in GLOBAL_KINDS:
+        xn, _ = orc.propose_global(KIND_ID[kind], O.Ran2Stream(11))
+        orc.set_fields(xn)
+        orc.init_expkv()
+        wo = orc.cal_new_weight()
+        print(f"ft global weight L={L} Lt={Lt} ng={ng} kind={kind}", flush=True)
+        wg = ch.global_weight(xn)
+        assert same_lncw(wg, wo), (kind, wg, wo)
+    assert np.array_equal(ch.get_fields(), x)
+    assert np.array_equal(ch.g_h, g0)
+    ch.close()
+
+
+@pytest.mark.gpu
+def test_gpu_ft_sweeps_with_global_moves_match_oracle():
+    """Four loops with the global update every second loop (a small move
+    distance, so moves are accepted and init_g runs in the loop): identical
+    global and local accepts, fields; G to 1e-9."""
+    from pqmc_allinone_amd.host import GLOBAL_KINDS, GlobalMoves, Ran2
+    from test_global_moves import KIND_ID
+    L, Lt, ng, dist = 4, 40, 10, 0.05
+    m, x, orc = ft_pair(L, Lt, ng, seed=77)
+    orc.init(1)
+    ch = ft_chain(m, x)
+    ch.init_g(1)
+    rg, ro = Ran2(2024), O.Ran2Stream(2024)
+    gm = GlobalMoves(ch, rg, dist)
+    forward = True
+    for loop in range(1, 5):
+        forward = not forward
+        if not forward and (loop + 1) % 2 == 0:
+            print(f"ft global loop {loop}", flush=True)
+            flags_g = gm.run()
+            flags_o = [orc.update_global(KIND_ID[k], ro, dist) for k in GLOBAL_KINDS]
+            if any(flags_o):
+                orc.init(1)
+            assert flags_g == flags_o, loop
+        u = rg.draws(2 * m.Nb * Lt)
+        ro.draws(2 * m.Nb * Lt)
+        print(f"ft sweep loop {loop} forward={forward}", flush=True)
+        ch.sweep(forward, u)
+        acc_o, _ = orc.sweep(forward, u)
+        assert np.array_equal(ch.accepts(), acc_o), loop
+        assert np.array_equal(ch.get_fields(), orc.get_fields()), loop
+        assert rel(ch.g_h, orc.g) < 1e-9, loop
+    assert gm.accepted > 0 and gm.accepted + gm.rejected == 6
     ch.close()

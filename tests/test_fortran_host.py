@@ -30,7 +30,7 @@ def test_driver_input_layout_round_trip(tmp_path):
     host.write_driver_input(str(path), m, x, sl, nsweeps=2, seed=1234)
     npla, Ns, Nb, Lt, ne = m.npla, m.Ns, m.Nb, m.p.Lt, m.p.ne
     expect = (8 * 4 + 4 * npla * 4 * 4 + 2 * 4 * npla * 4 + 4 * npla * 16 * 16 + (4 + 4 + 16 + 16 + 2) * 16
-              + 7 * 8 + 2 * 4 + Nb * Lt * 8 + Ns * ne * 16 + 4 + 8 + 7 * 4)
+              + 7 * 8 + 2 * 4 + Nb * Lt * 8 + Ns * ne * 16 + 4 + 8 + 8 * 4)
     assert path.stat().st_size == expect
 
 
@@ -65,23 +65,25 @@ def test_fortran_global_proposals_match_oracle(tmp_path, kind):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(host.FORTRAN_DRIVER), reason="Fortran driver not built (amdflang missing)")
-@pytest.mark.parametrize("nb,glob", [(0, False), (3, False), (0, True)])
-def test_fortran_driver_matches_oracle(tmp_path, nb, glob):
+@pytest.mark.parametrize("nb,glob,ft", [(0, False, False), (3, False, False), (0, True, False), (0, False, True),
+                                        (0, True, True)])
+def test_fortran_driver_matches_oracle(tmp_path, nb, glob, ft):
     """glob: global_update_loop = 2 with a small move distance, so that global
-    moves are accepted and init_g(1) runs inside the loop."""
+    moves are accepted and init_g(1) runs inside the loop.  ft: proj =
+    .false., the finite-temperature Green's function."""
     L, Lt, ng, ne, seed = 4, 40, 10, 16, 1234
-    nsweeps = 4 if glob else 2
+    nsweeps = 4 if glob or ft else 2
     dist = 0.05
     m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne))
     x, sl = synthetic_inputs(m, seed)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     host.write_driver_input(str(fin), m, x, sl, nsweeps=nsweeps, seed=seed, block_bonds=nb,
-                            global_update_loop=2 if glob else 0, global_distance=dist)
+                            global_update_loop=2 if glob else 0, global_distance=dist, proj=not ft)
     r = subprocess.run([host.FORTRAN_DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = host.read_driver_output(str(fout), m, nsweeps)
 
-    orc = O.Oracle(L, L, Lt, ng, ne)
+    orc = O.Oracle(L, L, Lt, ng, ne, ft=ft)
     orc.set_fields(x)
     orc.set_trial(sl)
     orc.init(1)
