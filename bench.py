@@ -114,6 +114,42 @@ def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_
             "ms_per_step": 1e3 * dt / steps, "dtype": "c128 (fp64)", "max_abs_g_err_vs_plain": g_err}
 
 
+def ft_leg(m, x, seed, warmup, steps):
+    """The finite-temperature Green's function (proj = .false.) on the same
+    lattice, fields and draws: one chain, sweeps with a measurement at Lt/2;
+    G = (1 + B)^-1 from Ns x Ns strings, so each stabilization is heavier."""
+    from pqmc_allinone_amd.host import Chain, Ran2
+    ch = Chain(m, finite_temperature=True)
+    ch.set_fields(x)
+    t0 = time.perf_counter()
+    ch.init_g(1)
+    t_init = time.perf_counter() - t0
+    rng = Ran2(seed)
+    nd = 2 * m.Nb * m.p.Lt
+    fw = True
+    for _ in range(warmup):
+        fw = not fw
+        ch.sweep(fw, rng.draws(nd), measure=True)
+    for k in range(3):
+        ch.timing(k, True)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fw = not fw
+        ch.sweep(fw, rng.draws(nd), measure=True)
+    ch.synchronize()
+    dt = time.perf_counter() - t0
+    kern = {}
+    for k, name in enumerate(("family_pass", "wrap", "stabilization")):
+        ms, n = ch.timing(k, False)
+        kern[name] = {"ms_total": ms, "launches": n, "us_avg": 1e3 * ms / max(n, 1)}
+    out = {"value": steps / dt, "unit": "chain-sweeps/s", "ms_per_step": 1e3 * dt / steps,
+           "init_ms": 1e3 * t_init, "dtype": "c128 (fp64)", "kernels": kern,
+           "ln_cw_re": ch.ln_cw.real}
+    ch.close()
+    return out
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -142,6 +178,8 @@ def main():
                     help="chains of the batched leg (0 or 1: no batched leg)")
     ap.add_argument("--batch-helper-bonds", type=int, default=2,
                     help="bonds per column-owning workgroup in the batched leg's family passes (1 or 2)")
+    ap.add_argument("--ft-steps", type=int, default=2,
+                    help="timed sweeps of the finite-temperature leg (0: no such leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
@@ -222,6 +260,9 @@ def main():
         batched = batched_leg(m, x, sl, seed, args.batch, args.warmup, args.steps, args.block_bonds,
                               args.batch_helper_bonds, ch.g_h)
         batched["vs_plain"] = batched["value"] / sps
+    ft = None
+    if args.ft_steps > 0:
+        ft = ft_leg(m, x, seed, args.warmup, args.ft_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
@@ -246,6 +287,7 @@ def main():
         "acceptance": n_acc / max(n_prop, 1),
         "kernels": kern,
         "batched": batched,
+        "finite_temperature": ft,
         "cpu_baseline": cpu,
         "speedup_vs_cpu_core": (sps / cpu["value"]) if cpu else None,
     }

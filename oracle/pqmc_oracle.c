@@ -666,8 +666,8 @@ void orc_set_trial(orc_model *m, const double *slater) {
  * B(t,0) = Q D T from column-scaled QR (Q' = Q*D, then a UDV step), left
  * strings B(beta,t) = T D Q from row-scaled LQ, and G from the scale-split
  * formula of cal_g_and_det_FT (:494-648).  The reference's UDV is an SVD
- * (ZGESVD, src/mod_matrixlib.f90:704-743); a QR-based UDV gives the same G
- * and the same Re ln det(1 + B), which is all that enters the sweep. */
+ * (ZGESVD, src/mod_matrixlib.f90:704-743); a pre-pivoted QR (ft_prepivot)
+ * gives the same G and ln det(1 + B) with graded scales, as the SVD does. */
 
 static zc *FQS(orc_model *m, int b) { return m->FQ + (size_t)m->Ns * m->Ns * (b - 1); }
 static zc *FDS(orc_model *m, int b) { return m->FD + (size_t)m->Ns * (b - 1); }
@@ -694,34 +694,129 @@ static void ft_identity(orc_model *m, int b) {
   }
 }
 
-/* right string of block b from block b-1 and the slices t0..t1 */
+/* One string step, right (B Q D) or left (D Q B), factored so that no
+ * intermediate squares a scale: scales of B(beta,0) reach exp(beta * band
+ * edge) (exp(408) at beta = 100 here), and a Householder norm or a complex
+ * division of such a number overflows.
+ *
+ * - Each scale splits exactly as D_j = Dh_j 2^e_j with |Dh_j| in [0.5, 1)
+ *   (frexp); the factorization runs on X Dh (moderate entries), and the
+ *   powers of two go back exactly afterwards.  Householder QR is invariant
+ *   under power-of-two column scaling, so this is the QR of X D.
+ * - Pre-pivoting (Bai, Lee, Li, Xu, "Stable solutions of linear systems
+ *   involving long chain of matrix multiplications", Linear Algebra Appl.
+ *   435 (2011) 659-673): the columns of X D are ordered by decreasing
+ *   2-norm before the unpivoted QR, so that the new D comes out graded and
+ *   T = D^-1 R stays bounded.  The reference's UDV is an SVD, whose D is
+ *   sorted; an unpivoted QR leaves ratios of scales in T.
+ *
+ * Squared norms are compared as 4^e_j v_j with v_j = |Dh_j|^2 sum_i |X_ij|^2,
+ * through ldexp (exact), ties by index.  perm[k] = the column taken to
+ * position k, ep[k] = e_perm[k]. */
+static void ft_prepivot(int n, const double *v, const int *ex, int *perm, int *ep) {
+  for (int j = 0; j < n; ++j) {
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      double vi = ldexp(v[i], 2 * (ex[i] - ex[j]));
+      rank += (vi > v[j]) || (vi == v[j] && i < j);
+    }
+    perm[rank] = j;
+    ep[rank] = ex[j];
+  }
+}
+
+/* v_j and e_j of column j (left: row j) of X against the scales D */
+static void ft_keys(int n, const zc *x, const zc *D, int left, double *v, int *ex) {
+  for (int j = 0; j < n; ++j) {
+    double ss = 0.0;
+    for (int i = 0; i < n; ++i) {
+      zc a = left ? x[j + (size_t)n * i] : x[i + (size_t)n * j];
+      ss += creal(a) * creal(a) + cimag(a) * cimag(a);
+    }
+    double m = frexp(cabs(D[j]), &ex[j]);
+    v[j] = m * m * ss;
+  }
+}
+
+/* right string of block b from block b-1 and the slices t0..t1:
+ * B Q_{b-1} D_{b-1} P = Q_b D_b T'; T_b = T' P^T T_{b-1} */
 static void ft_right(orc_model *m, int b, int t0, int t1) {
   int Ns = m->Ns;
   zc *x = xcalloc((size_t)Ns * Ns, sizeof(zc)), *r = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  zc *xp = xcalloc((size_t)Ns * Ns, sizeof(zc)), *tg = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  double *v = xcalloc(Ns, sizeof(double));
+  int *ex = xcalloc(Ns, sizeof(int)), *perm = xcalloc(Ns, sizeof(int)), *ep = xcalloc(Ns, sizeof(int));
+  const zc *D = FDS(m, b - 1);
   memcpy(x, FQS(m, b - 1), sizeof(zc) * Ns * Ns);
   for (int p = t0; p <= t1; ++p)
     for (int f = 1; f <= 4; ++f) orc_left_evolve(m, p, (double *)x, Ns, f, 0);
-  for (int j = 0; j < Ns; ++j)
-    for (int i = 0; i < Ns; ++i) x[i + (size_t)Ns * j] *= FDS(m, b - 1)[j]; /* Q' = Q*D */
-  zqdr(Ns, Ns, x, r, FDS(m, b));
-  memcpy(FQS(m, b), x, sizeof(zc) * Ns * Ns);
-  gemm_nn(Ns, Ns, Ns, r, FTS(m, b - 1), FTS(m, b)); /* T_b = T' T_{b-1} */
+  ft_keys(Ns, x, D, 0, v, ex);
+  ft_prepivot(Ns, v, ex, perm, ep);
+  for (int k = 0; k < Ns; ++k) { /* (X D P) 2^-ep: column k = X(:, perm[k]) Dh_perm[k] */
+    zc dh = CMPLX(ldexp(creal(D[perm[k]]), -ep[k]), ldexp(cimag(D[perm[k]]), -ep[k]));
+    for (int i = 0; i < Ns; ++i) xp[i + (size_t)Ns * k] = x[i + (size_t)Ns * perm[k]] * dh;
+  }
+  zqdr(Ns, Ns, xp, r, FDS(m, b));
+  for (int k = 0; k < Ns; ++k) {
+    zc *d = &FDS(m, b)[k];
+    *d = CMPLX(ldexp(creal(*d), ep[k]), ldexp(cimag(*d), ep[k]));
+    for (int j = k + 1; j < Ns; ++j) {
+      zc *t = &r[k + (size_t)Ns * j];
+      *t = CMPLX(ldexp(creal(*t), ep[j] - ep[k]), ldexp(cimag(*t), ep[j] - ep[k]));
+    }
+  }
+  memcpy(FQS(m, b), xp, sizeof(zc) * Ns * Ns);
+  const zc *tp = FTS(m, b - 1);
+  for (int j = 0; j < Ns; ++j) /* rows of T_{b-1} in pivot order: P^T T_{b-1} */
+    for (int k = 0; k < Ns; ++k) tg[k + (size_t)Ns * j] = tp[perm[k] + (size_t)Ns * j];
+  gemm_nn(Ns, Ns, Ns, r, tg, FTS(m, b));
+  free(ep);
+  free(perm);
+  free(ex);
+  free(v);
+  free(tg);
+  free(xp);
   free(r);
   free(x);
 }
 
-/* left string of block b from block b+1 and the slices t1 down to t0 */
+/* left string of block b from block b+1 and the slices t1 down to t0, rows
+ * pre-pivoted: P^T D_{b+1} Q_{b+1} B = L' D_b Q_b; T_b = T_{b+1} P L' */
 static void ft_left(orc_model *m, int b, int t0, int t1) {
   int Ns = m->Ns;
   zc *y = xcalloc((size_t)Ns * Ns, sizeof(zc)), *l = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  zc *yp = xcalloc((size_t)Ns * Ns, sizeof(zc)), *tg = xcalloc((size_t)Ns * Ns, sizeof(zc));
+  double *v = xcalloc(Ns, sizeof(double));
+  int *ex = xcalloc(Ns, sizeof(int)), *perm = xcalloc(Ns, sizeof(int)), *ep = xcalloc(Ns, sizeof(int));
+  const zc *D = FDS(m, b + 1);
   memcpy(y, FQS(m, b + 1), sizeof(zc) * Ns * Ns);
   for (int p = t1; p >= t0; --p)
     for (int f = 4; f >= 1; --f) orc_right_evolve(m, p, (double *)y, Ns, f, 0);
-  for (int j = 0; j < Ns; ++j)
-    for (int i = 0; i < Ns; ++i) y[i + (size_t)Ns * j] *= FDS(m, b + 1)[i]; /* rows scaled by D */
-  zldq(Ns, Ns, y, l, FDS(m, b));
-  memcpy(FQS(m, b), y, sizeof(zc) * Ns * Ns);
-  gemm_nn(Ns, Ns, Ns, FTS(m, b + 1), l, FTS(m, b)); /* T_b = T_{b+1} T' */
+  ft_keys(Ns, y, D, 1, v, ex);
+  ft_prepivot(Ns, v, ex, perm, ep);
+  for (int k = 0; k < Ns; ++k) { /* (P^T D Q B) 2^-ep: row k = Dh_perm[k] Y(perm[k], :) */
+    zc dh = CMPLX(ldexp(creal(D[perm[k]]), -ep[k]), ldexp(cimag(D[perm[k]]), -ep[k]));
+    for (int j = 0; j < Ns; ++j) yp[k + (size_t)Ns * j] = dh * y[perm[k] + (size_t)Ns * j];
+  }
+  zldq(Ns, Ns, yp, l, FDS(m, b));
+  for (int k = 0; k < Ns; ++k) {
+    zc *d = &FDS(m, b)[k];
+    *d = CMPLX(ldexp(creal(*d), ep[k]), ldexp(cimag(*d), ep[k]));
+    for (int i = k + 1; i < Ns; ++i) {
+      zc *t = &l[i + (size_t)Ns * k];
+      *t = CMPLX(ldexp(creal(*t), ep[i] - ep[k]), ldexp(cimag(*t), ep[i] - ep[k]));
+    }
+  }
+  memcpy(FQS(m, b), yp, sizeof(zc) * Ns * Ns);
+  const zc *tn = FTS(m, b + 1);
+  for (int k = 0; k < Ns; ++k) memcpy(tg + (size_t)Ns * k, tn + (size_t)Ns * perm[k], sizeof(zc) * Ns);
+  gemm_nn(Ns, Ns, Ns, tg, l, FTS(m, b)); /* T_b = (T_{b+1} P) L' */
+  free(ep);
+  free(perm);
+  free(ex);
+  free(v);
+  free(tg);
+  free(yp);
   free(l);
   free(y);
 }

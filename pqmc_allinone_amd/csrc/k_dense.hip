@@ -657,14 +657,88 @@ extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const d
 PQ_DEV zd ft_big(zd d) { return (d.x * d.x + d.y * d.y > 1.0) ? d : zmk(1.0, 0.0); }
 PQ_DEV zd ft_small(zd d) { return (d.x * d.x + d.y * d.y > 1.0) ? zmk(1.0, 0.0) : d; }
 
-// X (m x n, ld): mode 0 multiplies column j by D[j], mode 2 by conj(D[j]);
-// mode 1 divides row i by the big part of D[i]
-__global__ void k_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode) {
+// Scales of B(beta,0) reach exp(beta * band edge) (about exp(400) at
+// beta = 100), so nothing here squares one: a / d rescales d by a power of
+// two first, and the string steps factor X Dh with D = Dh 2^e, |Dh| in
+// [0.5, 1), restoring the powers of two exactly afterwards.
+PQ_DEV zd zdiv_scaled(zd a, zd d) {
+  int e;
+  frexp(fmax(fabs(d.x), fabs(d.y)), &e);
+  const zd q = zdiv(a, zmk(ldexp(d.x, -e), ldexp(d.y, -e)));
+  return zmk(ldexp(q.x, -e), ldexp(q.y, -e));
+}
+
+// X (n x n): row i divided by the big part of D[i]
+__global__ void k_ft_rowdiv(zd *X, int n, const zd *D) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= (long)m * n) return;
-  const int i = (int)(e % m), j = (int)(e / m);
-  zd &x = X[i + (size_t)ld * j];
-  x = mode == 1 ? zdiv(x, ft_big(D[i])) : zmul(x, mode == 2 ? zconj(D[j]) : D[j]);
+  if (e >= (long)n * n) return;
+  X[e] = zdiv_scaled(X[e], ft_big(D[e % n]));
+}
+
+// Pre-pivoting of one string step (one workgroup of 1024; Bai, Lee, Li, Xu,
+// Linear Algebra Appl. 435 (2011) 659-673): column j of X D has squared
+// norm 4^e_j v_j with |D_j| = a_j 2^e_j (frexp) and v_j = a_j^2 sum_i
+// |X_ij|^2; perm orders the columns by it, decreasing, ties by index
+// (compared through ldexp, exact); ep[k] = e_perm[k].
+__global__ __launch_bounds__(1024) void k_ft_prep(const zd *X, int n, const zd *D, int *perm, int *ep) {
+  __shared__ double v[1024];
+  __shared__ int ex[1024];
+  const int tid = threadIdx.x, lane = tid & 63, nw = blockDim.x >> 6;
+  for (int j = tid >> 6; j < n; j += nw) {
+    double ss = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const zd x = X[i + (size_t)n * j];
+      ss += x.x * x.x + x.y * x.y;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) {
+      int e;
+      const double a = frexp(hypot(D[j].x, D[j].y), &e);
+      v[j] = a * a * ss;
+      ex[j] = e;
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += blockDim.x) {
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const double vi = ldexp(v[i], 2 * (ex[i] - ex[j]));
+      rank += (vi > v[j]) || (vi == v[j] && i < j);
+    }
+    perm[rank] = j;
+    ep[rank] = ex[j];
+  }
+}
+
+// Xp(:, k) = X(:, perm[k]) Dh_perm[k], Dh = D 2^-e (its conjugate if conj)
+__global__ void k_ft_gather_scale(const zd *X, zd *Xp, int n, const zd *D, const int *perm, const int *ep,
+                                  int conj) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e % n), k = (int)(e / n), j = perm[k];
+  zd dh = zmk(ldexp(D[j].x, -ep[k]), ldexp(D[j].y, -ep[k]));
+  if (conj) dh = zconj(dh);
+  Xp[e] = zmul(X[i + (size_t)n * j], dh);
+}
+
+// The powers of two back after the QR of Xp: D[k] *= 2^ep[k]; the unit
+// triangular factor T(r, c) *= 2^(ep[c] - ep[r]) (conjt, its conjugate
+// transpose: 2^(ep[r] - ep[c]))
+__global__ void k_ft_unscale(zd *T, zd *D, int n, const int *ep, int conjt) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const int s = conjt ? ep[r] - ep[c] : ep[c] - ep[r];
+  T[e] = zmk(ldexp(T[e].x, s), ldexp(T[e].y, s));
+  if (e < n) D[e] = zmk(ldexp(D[e].x, ep[e]), ldexp(D[e].y, ep[e]));
+}
+
+// rows: dst(k, :) = src(perm[k], :); else dst(:, k) = src(:, perm[k])
+__global__ void k_ft_permute(const zd *src, zd *dst, int n, const int *perm, int rows) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int i = (int)(e % n), k = (int)(e / n);
+  dst[e] = rows ? src[perm[i] + (size_t)n * k] : src[i + (size_t)n * perm[k]];
 }
 
 // M = db_r^-1 QQ db_l^-1 + ds_r TT ds_l  (n x n, ld n)
@@ -672,7 +746,7 @@ __global__ void k_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const 
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= (long)n * n) return;
   const int i = (int)(e % n), j = (int)(e / n);
-  const zd a = zdiv(zdiv(QQ[e], ft_big(Dr[i])), ft_big(Dl[j]));
+  const zd a = zdiv_scaled(zdiv_scaled(QQ[e], ft_big(Dr[i])), ft_big(Dl[j]));
   const zd b = zmul(zmul(ft_small(Dr[i]), TT[e]), ft_small(Dl[j]));
   M[e] = zadd(a, b);
 }
@@ -745,8 +819,26 @@ __global__ void k_ft_finish(zd *G, int n) {
 
 static inline int ft_grid(long n) { return (int)((n + 255) / 256); }
 
-extern "C" hipError_t pqmc_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode, hipStream_t s) {
-  hipLaunchKernelGGL(k_ft_scale, dim3(ft_grid((long)m * n)), dim3(256), 0, s, X, ld, m, n, D, mode);
+extern "C" hipError_t pqmc_ft_rowdiv(zd *X, int n, const zd *D, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_rowdiv, dim3(ft_grid((long)n * n)), dim3(256), 0, s, X, n, D);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_prep(const zd *X, int n, const zd *D, int *perm, int *ep, hipStream_t s) {
+  if (n > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ft_prep, dim3(1), dim3(1024), 0, s, X, n, D, perm, ep);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_gather_scale(const zd *X, zd *Xp, int n, const zd *D, const int *perm, const int *ep,
+                                           int conj, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_gather_scale, dim3(ft_grid((long)n * n)), dim3(256), 0, s, X, Xp, n, D, perm, ep, conj);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_unscale(zd *T, zd *D, int n, const int *ep, int conjt, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_unscale, dim3(ft_grid((long)n * n)), dim3(256), 0, s, T, D, n, ep, conjt);
+  return hipGetLastError();
+}
+extern "C" hipError_t pqmc_ft_permute(const zd *src, zd *dst, int n, const int *perm, int rows, hipStream_t s) {
+  hipLaunchKernelGGL(k_ft_permute, dim3(ft_grid((long)n * n)), dim3(256), 0, s, src, dst, n, perm, rows);
   return hipGetLastError();
 }
 extern "C" hipError_t pqmc_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const zd *Dl, int n,

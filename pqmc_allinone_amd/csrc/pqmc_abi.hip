@@ -57,7 +57,12 @@ hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStr
 hipError_t pqmc_mfma_probe(const double *A, const double *B, double *C, hipStream_t s);
 hipError_t pqmc_launch_measure(const model_t *md, const zd *G, const double *x, int time, const zd *phase,
                                zd *out, hipStream_t s);
-hipError_t pqmc_ft_scale(zd *X, int ld, int m, int n, const zd *D, int mode, hipStream_t s);
+hipError_t pqmc_ft_rowdiv(zd *X, int n, const zd *D, hipStream_t s);
+hipError_t pqmc_ft_prep(const zd *X, int n, const zd *D, int *perm, int *ep, hipStream_t s);
+hipError_t pqmc_ft_gather_scale(const zd *X, zd *Xp, int n, const zd *D, const int *perm, const int *ep, int conj,
+                                hipStream_t s);
+hipError_t pqmc_ft_unscale(zd *T, zd *D, int n, const int *ep, int conjt, hipStream_t s);
+hipError_t pqmc_ft_permute(const zd *src, zd *dst, int n, const int *perm, int rows, hipStream_t s);
 hipError_t pqmc_ft_mix(zd *M, const zd *QQ, const zd *TT, const zd *Dr, const zd *Dl, int n, hipStream_t s);
 hipError_t pqmc_ft_dlog(const zd *Dr, const zd *Dl, const zd *Lr, const zd *Ll, int n, zd *out, hipStream_t s);
 hipError_t pqmc_ft_qlog(const zd *tau, const double *vn2, int n, int conj, zd *out, hipStream_t s);
@@ -137,6 +142,7 @@ struct pqmc_handle {
   zd *d_FQ = nullptr, *d_FT = nullptr, *d_FD = nullptr, *d_FL = nullptr;
   zd *d_fw = nullptr;    // five Ns x Ns work matrices
   zd *d_flog = nullptr;  // [4] log terms of ln det(1 + B)
+  int *d_fperm = nullptr;  // [2 Ns] pre-pivoting order and scale exponents of a string step
   qr_ws qf;              // QR workspace for Ns x Ns
   // timing
   bool timing[3] = {false, false, false};
@@ -295,6 +301,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
     A_(h->d_FL, h->nblock);
     A_(h->d_fw, 5 * nn);
     A_(h->d_flog, 4);
+    A_(h->d_fperm, 2 * Ns);
     A_(h->qf.V, nn);
     A_(h->qf.T, (size_t)16 * 16 * ((Ns + 7) / 8));
     A_(h->qf.tau, Ns);
@@ -343,7 +350,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
-                  h->d_flog, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft};
+                  h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -546,24 +553,41 @@ static int ft_identity(pqmc_handle *h, int b) {
   return PQMC_OK;
 }
 
-// one right-string step over slices t0..t1 with the exponentials ekv:
-// B Q D = Q' D' T'' (column-scaled QR), T' = T'' T; L' = log det Q'
+// One string step over slices t0..t1 with the exponentials ekv.
+//  right: B Q D P = Q' D' T'';            T' = T'' P^T T
+//  left (kept as conjugate transposes): (P^T D Q B)^H = Q'^H D'^H L''^H;
+//         T' = T P L''  (L'' unit lower)
+// Scales reach exp(400) at beta = 100: the QR runs on X Dh with
+// D = Dh 2^e, |Dh| in [0.5, 1), and the powers of two go back exactly into
+// D' and T'' (Householder QR is invariant under power-of-two column
+// scaling).  P orders the columns of X D by decreasing norm before the
+// unpivoted QR (pre-pivoting, Bai, Lee, Li, Xu, Linear Algebra Appl. 435
+// (2011) 659-673), so that D' is graded and T'' bounded, as the sorted
+// scales of the reference's SVD-based udv (src/mod_matrixlib.f90:704-743)
+// are.  L' = log det of the new unitary factor.
 struct ft_udt {
   zd *Q, *D, *T, *L;
 };
 
-static int ft_right_step(pqmc_handle *h, ft_udt in, ft_udt out, int t0, int t1, const ekv_t *ekv) {
+static int ft_step(pqmc_handle *h, ft_udt in, ft_udt out, int t0, int t1, const ekv_t *ekv, int left) {
   const int Ns = h->Ns;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  zd *X = FW(h, 0), *Tp = FW(h, 1);
+  zd *X = FW(h, 0), *Tp = FW(h, 1), *Xp = FW(h, 2), *Tg = FW(h, 3);
+  int *perm = h->d_fperm, *ep = h->d_fperm + Ns;
   HIPCHK(hipMemcpyAsync(X, in.Q, sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
-  int rc = propagate(h, X, t0, t1 - t0 + 1, 0, ekv, Ns);
+  int rc = propagate(h, X, left ? t1 : t0, t1 - t0 + 1, left, ekv, Ns);
   if (rc) return rc;
-  HIPCHK(pqmc_ft_scale(X, Ns, Ns, Ns, in.D, 0, h->st));
-  rc = run_qr(h, h->qf, X, Ns, Ns, Ns, out.Q, out.D, Tp, 0);
+  HIPCHK(pqmc_ft_prep(X, Ns, in.D, perm, ep, h->st));
+  HIPCHK(pqmc_ft_gather_scale(X, Xp, Ns, in.D, perm, ep, left, h->st));
+  rc = run_qr(h, h->qf, Xp, Ns, Ns, Ns, out.Q, out.D, Tp, left);
   if (rc) return rc;
-  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 0, out.L, h->st));
-  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tp, Ns, in.T, Ns, zero, out.T, Ns, h->st));
+  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, left, out.L, h->st));
+  HIPCHK(pqmc_ft_unscale(Tp, out.D, Ns, ep, left, h->st));
+  HIPCHK(pqmc_ft_permute(in.T, Tg, Ns, perm, !left, h->st));
+  if (left)
+    HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tg, Ns, Tp, Ns, zero, out.T, Ns, h->st));
+  else
+    HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Tp, Ns, Tg, Ns, zero, out.T, Ns, h->st));
   return PQMC_OK;
 }
 
@@ -571,24 +595,12 @@ static ft_udt ft_block(pqmc_handle *h, int b) { return {FQS(h, b), FDS(h, b), FT
 
 // right string of block b from block b-1 and slices t0..t1
 static int ft_right(pqmc_handle *h, int b, int t0, int t1) {
-  return ft_right_step(h, ft_block(h, b - 1), ft_block(h, b), t0, t1, h->d_ekv);
+  return ft_step(h, ft_block(h, b - 1), ft_block(h, b), t0, t1, h->d_ekv, 0);
 }
 
-// left string of block b from block b+1 and slices t1 down to t0, kept as
-// conjugate transposes: (D_{b+1} Q_{b+1} B)^H = Q_b^H D_b L'^H; T_b = T_{b+1} L'
+// left string of block b from block b+1 and slices t1 down to t0
 static int ft_left(pqmc_handle *h, int b, int t0, int t1) {
-  const int Ns = h->Ns;
-  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  zd *W = FW(h, 0), *Lp = FW(h, 1);
-  HIPCHK(hipMemcpyAsync(W, FQS(h, b + 1), sizeof(zd) * Ns * Ns, hipMemcpyDeviceToDevice, h->st));
-  int rc = propagate(h, W, t1, t1 - t0 + 1, 1, nullptr, Ns);
-  if (rc) return rc;
-  HIPCHK(pqmc_ft_scale(W, Ns, Ns, Ns, FDS(h, b + 1), 2, h->st));
-  rc = run_qr(h, h->qf, W, Ns, Ns, Ns, FQS(h, b), FDS(h, b), Lp, 1);
-  if (rc) return rc;
-  HIPCHK(pqmc_ft_qlog(h->qf.tau, h->qf.vn2, Ns, 1, h->d_FL + (b - 1), h->st));
-  HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, FTS(h, b + 1), Ns, Lp, Ns, zero, FTS(h, b), Ns, h->st));
-  return PQMC_OK;
+  return ft_step(h, ft_block(h, b + 1), ft_block(h, b), t0, t1, h->d_ekv, 1);
 }
 
 // M = Db_r^-1 Q_r^H Q_l^H Db_l^-1 + Ds_r T_r T_l Ds_l in FW(2), factored by
@@ -620,10 +632,10 @@ static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout) {
   if (rc) return rc;
   // TT <- Db_r^-1 Q_r^H;  Z = R_M^-1 Q_M^H TT;  Z <- Db_l^-1 Z;  G = Q_l^H Z
   HIPCHK(pqmc_conjt(Qr, Ns, Ns, Ns, TT, Ns, h->st));
-  HIPCHK(pqmc_ft_scale(TT, Ns, Ns, Ns, Dr, 1, h->st));
+  HIPCHK(pqmc_ft_rowdiv(TT, Ns, Dr, h->st));
   HIPCHK(pqmc_zgemm(1, 0, Ns, Ns, Ns, one, QM, Ns, TT, Ns, zero, Z, Ns, h->st));
   HIPCHK(pqmc_trsm_upper(M, Ns, Ns, Z, Ns, Ns, h->d_rinv, h->st));
-  HIPCHK(pqmc_ft_scale(Z, Ns, Ns, Ns, Dl, 1, h->st));
+  HIPCHK(pqmc_ft_rowdiv(Z, Ns, Dl, h->st));
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, Ns, one, Ul, Ns, Z, Ns, zero, Gout, Ns, h->st));
   HIPCHK(pqmc_ft_finish(Gout, Ns, h->st));
   HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
@@ -1084,7 +1096,7 @@ static int ft_global_weight(pqmc_handle *h, const double *x, double *ri) {
   HIPCHK(hipMemsetAsync(id.L, 0, sizeof(zd), h->st));
   ft_udt cur = id;
   for (int b = 2; b <= nb - 1; ++b) {
-    rc = ft_right_step(h, cur, buf[b & 1], (b - 2) * ng + 1, (b - 1) * ng, h->d_gekv);
+    rc = ft_step(h, cur, buf[b & 1], (b - 2) * ng + 1, (b - 1) * ng, h->d_gekv, 0);
     if (rc) return rc;
     cur = buf[b & 1];
   }

@@ -62,6 +62,56 @@ def check_lncw_dense(m, x, lncw):
     assert abs(dphi) < 1e-9 + (m.Ns + 3) / 2 * 1.8e-7
 
 
+def qrp_chain_g(m, x):
+    """G = (1 + B(beta,0))^-1, Re ln det(1 + B) and the log of the largest
+    scale, from an independent UDT chain: LAPACK's column-pivoted QR
+    (ZGEQP3 through scipy, overflow-safe norms) per ngroup slices,
+    Q D T <- QRP(B_blk Q D) with T accumulated, then the scale-split formula
+    of cal_g_and_det_FT with the identity as the left string.  (A chain of
+    plain SVDs, the reference's udv, loses the small scales of these graded
+    products and is not used as a check.)"""
+    import scipy.linalg as sla
+    Ns, Lt, ng = m.Ns, m.p.Lt, m.p.ngroup
+    Q, d, Tm = np.eye(Ns, dtype=complex), np.ones(Ns, dtype=complex), np.eye(Ns, dtype=complex)
+    for b in range(Lt // ng):
+        X = np.eye(Ns, dtype=complex)
+        for t in range(b * ng + 1, (b + 1) * ng + 1):
+            X = BF.slice_matrix(m, x, t) @ X
+        q, r, piv = sla.qr((X @ Q) * d, pivoting=True)
+        d = np.diag(r).copy()
+        t = np.empty_like(r)
+        t[:, piv] = r / d[:, None]  # T' P^T
+        Q, Tm = q, t @ Tm
+    ad = np.abs(d)
+    db = np.where(ad > 1, d, 1.0)
+    ds = np.where(ad > 1, 1.0, d)
+    # 1 + Q D T = Q Db (Db^-1 Q^H T^-1 + Ds) T
+    Mm = np.linalg.solve(Tm.T, Q.conj()).T / db[:, None] + np.diag(ds)
+    G = np.linalg.solve(Tm, np.linalg.solve(Mm, Q.conj().T / db[:, None]))
+    lw = np.log(np.abs(db)).sum() + np.linalg.slogdet(Mm)[1] + np.linalg.slogdet(Tm)[1]
+    return np.eye(Ns) - G, lw, np.log(ad.max())
+
+
+@pytest.mark.parametrize("L,Lt,ng,ng2", [(2, 1000, 10, 20), (4, 1000, 10, 25), (4, 500, 5, 1), (2, 100, 10, 1)])
+def test_ft_large_beta_matches_pivoted_qr_chain(L, Lt, ng, ng2):
+    """beta = 100, 50 and 10 (dense inverses are useless from about 10): at
+    beta = 100 the largest scale of B is about exp(400), so its square
+    overflows a double; the pre-pivoted, exactly rescaled QR strings give
+    the G and Re ln det(1 + B) of a column-pivoted QR chain, and the same G
+    for another block length ng2."""
+    m, x, orc = ft_pair(L, Lt, ng, seed=7)
+    orc.init(1)
+    ref, lw, lmax = qrp_chain_g(m, x)
+    if Lt == 1000:
+        assert lmax > 354
+    assert np.isfinite(orc.g).all()
+    assert np.abs(orc.g - ref).max() < 1e-9
+    assert abs(orc.ln_cw.real - lw) < 1e-10 * abs(lw)
+    _, _, o2 = ft_pair(L, Lt, ng2, seed=7)
+    o2.init(1)
+    assert np.abs(o2.g - orc.g).max() < 1e-12
+
+
 @pytest.mark.parametrize("L,Lt,ng", [(2, 30, 5), (4, 20, 5)])
 def test_ft_sweeps_match_dense(L, Lt, ng):
     """Two sweeps with updates: G at the end from the strings equals the dense
@@ -116,14 +166,17 @@ def ft_chain(m, x):
     return ch
 
 
-def same_lncw(a, b):
-    """Re to 1e-9 relative; Im to 1e-8 up to one 2 pi_SP of the final
-    reduction (the branches of every log term agree, so the unreduced sums
-    agree)."""
+def same_lncw(a, b, ns):
+    """Re to 1e-9 relative; Im as a phase.  det(1 + B) is real and positive
+    here and B is real at zero boundary phases, so det M, det Q and the
+    scales are real too: each log term is 0 or +-i pi, the sign of a
+    rounded zero choosing the branch, and the unreduced sums of the two
+    sides may differ by multiples of 2 pi.  The final mod 2 pi_SP turns
+    those into multiples of 2 pi - 2 pi_SP (1.75e-7), at most one per term."""
     if abs(a.real - b.real) > 1e-9 * max(1.0, abs(b.real)):
         return False
-    d = abs(a.imag - b.imag)
-    return min(d, abs(d - SP_TWO_PI)) < 1e-8
+    d = abs(a.imag - b.imag) % (2 * np.pi)
+    return min(d, 2 * np.pi - d) < 1e-9 + (ns + 3) / 2 * 1.8e-7
 
 
 @pytest.mark.gpu
@@ -137,7 +190,7 @@ def test_gpu_ft_init_matches_oracle(L, Lt, ng):
     e = rel(ch.g_h, orc.g)
     print(f"  rel G err {e:.3e}  lncw gpu {ch.ln_cw} oracle {orc.ln_cw}", flush=True)
     assert e < 1e-10
-    assert same_lncw(ch.ln_cw, orc.ln_cw)
+    assert same_lncw(ch.ln_cw, orc.ln_cw, m.Ns)
     ch.close()
 
 
@@ -165,7 +218,7 @@ def test_gpu_ft_sweeps_match_oracle(L, Lt, ng):
         print(f"  accepts {int(acc.sum())} rel G err {e:.3e} meas err {rel(mg, mo):.3e}", flush=True)
         assert e < 1e-9, sweep
         assert rel(mg, mo) < 1e-9, sweep
-        assert same_lncw(ch.ln_cw, orc.ln_cw), (sweep, ch.ln_cw, orc.ln_cw)
+        assert same_lncw(ch.ln_cw, orc.ln_cw, m.Ns), (sweep, ch.ln_cw, orc.ln_cw)
         tot += int(acc.sum())
     assert tot > 0
     ch.close()
@@ -252,7 +305,7 @@ def test_gpu_ft_global_weight_matches_oracle(L, Lt, ng):
         wo = orc.cal_new_weight()
         print(f"ft global weight L={L} Lt={Lt} ng={ng} kind={kind}", flush=True)
         wg = ch.global_weight(xn)
-        assert same_lncw(wg, wo), (kind, wg, wo)
+        assert same_lncw(wg, wo, m.Ns), (kind, wg, wo)
     assert np.array_equal(ch.get_fields(), x)
     assert np.array_equal(ch.g_h, g0)
     ch.close()
@@ -291,4 +344,35 @@ def test_gpu_ft_sweeps_with_global_moves_match_oracle():
         assert np.array_equal(ch.get_fields(), orc.get_fields()), loop
         assert rel(ch.g_h, orc.g) < 1e-9, loop
     assert gm.accepted > 0 and gm.accepted + gm.rejected == 6
+    ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,Lt,ng,nsweeps", [(4, 1000, 10, 2), (8, 1000, 10, 1)])
+def test_gpu_ft_large_beta_matches_oracle(L, Lt, ng, nsweeps):
+    """beta = 100, where the scales of B square past the double range: the
+    device's rescaled, pre-pivoted strings against the oracle's, at init and
+    over sweeps (identical accepts and fields, G to 1e-9)."""
+    m, x, orc = ft_pair(L, Lt, ng, seed=7)
+    orc.init(1)
+    ch = ft_chain(m, x)
+    print(f"ft large beta L={L} Lt={Lt} init", flush=True)
+    ch.init_g(1)
+    e = rel(ch.g_h, orc.g)
+    print(f"  rel G err {e:.3e} lncw {ch.ln_cw} / {orc.ln_cw}", flush=True)
+    assert e < 1e-9
+    assert same_lncw(ch.ln_cw, orc.ln_cw, m.Ns)
+    rng = O.Ran2Stream(3)
+    fw = True
+    for sweep in range(nsweeps):
+        fw = not fw
+        u = rng.draws(2 * m.Nb * Lt)
+        print(f"ft large beta L={L} sweep={sweep} forward={fw}", flush=True)
+        ch.sweep(fw, u)
+        acc_o, _ = orc.sweep(fw, u)
+        assert np.array_equal(ch.accepts(), acc_o), sweep
+        assert np.array_equal(ch.get_fields(), orc.get_fields()), sweep
+        e = rel(ch.g_h, orc.g)
+        print(f"  rel G err {e:.3e}", flush=True)
+        assert e < 1e-9, sweep
     ch.close()
