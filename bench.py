@@ -174,10 +174,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
-    ap.add_argument("--batch", type=int, default=5,
+    ap.add_argument("--batch", type=int, default=9,
                     help="chains of the batched leg (0 or 1: no batched leg)")
-    ap.add_argument("--batch-helper-bonds", type=int, default=2,
-                    help="bonds per column-owning workgroup in the batched leg's family passes (1 or 2)")
+    ap.add_argument("--batch-helper-bonds", type=int, default=4,
+                    help="bonds per column-owning workgroup in the batched leg's family passes (1, 2 or 4)")
     ap.add_argument("--ft-steps", type=int, default=2,
                     help="timed sweeps of the finite-temperature leg (0: no such leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

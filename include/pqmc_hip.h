@@ -41,9 +41,9 @@ typedef struct pqmc_config {
   int ngroup;         /* stabilization interval */
   int ne;             /* nelec */
   int block_bonds;    /* bonds per decision block of a family pass (1..8), 0 = automatic */
-  int helper_bonds;   /* bonds per column-owning workgroup of a family pass (1 or 2), 0 = automatic:
-                         1 when the pass fits the device.  2 halves the compute units a pass keeps
-                         resident, so that more chains fit a batch (pqmc_sweep_batch). */
+  int helper_bonds;   /* bonds per column-owning workgroup of a family pass (1, 2 or 4), 0 = automatic:
+                         1 when the pass fits the device.  2 and 4 divide the compute units a pass
+                         keeps resident, so that more chains fit a batch (pqmc_sweep_batch). */
   int finite_temperature;  /* 0: projector (proj = .true.); 1: finite temperature, G = (1 + B(beta,0))^-1
                               from Ns x Ns UDT strings (init_g_FT / get_g_scratch_FT,
                               src/mod_evolution.f90:411-785).  Needs Lt % ngroup == 0; no trial

@@ -1055,8 +1055,10 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     int *sS = reinterpret_cast<int *>(sC + 4 * A.bph * A.md.Ns);
     if (A.bph == 1)
       ps_helper<1>(A, Hh, sC, sw, sy, sS);
-    else
+    else if (A.bph == 2)
       ps_helper<2>(A, Hh, sC, sw, sy, sS);
+    else
+      ps_helper<4>(A, Hh, sC, sw, sy, sS);
   }
   // the last workgroup to finish clears the sync words for the next launch
   __syncthreads();
@@ -1085,7 +1087,7 @@ extern "C" size_t pqmc_pass_lds(int Ns, int bph) {
 extern "C" size_t pqmc_pass_rec_bytes(void) { return sizeof(ps_rec); }
 
 extern "C" hipError_t pqmc_launch_pass(const ps_args *a, hipStream_t s) {
-  if (a->bph < 1 || a->bph > 2) return hipErrorInvalidValue;
+  if (a->bph != 1 && a->bph != 2 && a->bph != 4) return hipErrorInvalidValue;
   const size_t lds = pqmc_pass_lds(a->md.Ns, a->bph);
   static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 0;

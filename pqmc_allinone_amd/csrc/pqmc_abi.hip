@@ -225,9 +225,12 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   {
     // every workgroup of a family pass must be resident at once: one leader
     // plus ceil(npla / bph) helpers
-    if (cfg->helper_bonds < 0 || cfg->helper_bonds > 2) { delete h; return fail(PQMC_EINVAL, "helper_bonds out of range"); }
+    if (cfg->helper_bonds != 0 && cfg->helper_bonds != 1 && cfg->helper_bonds != 2 && cfg->helper_bonds != 4) {
+      delete h;
+      return fail(PQMC_EINVAL, "helper_bonds must be 0, 1, 2 or 4");
+    }
     h->bph = 0;
-    for (int bph = cfg->helper_bonds ? cfg->helper_bonds : 1; bph <= 2 && !h->bph; ++bph) {
+    for (int bph = cfg->helper_bonds ? cfg->helper_bonds : 1; bph <= 4 && !h->bph; bph *= 2) {
       int maxg = 0;
       if (pqmc_pass_max_groups(h->Ns, bph, &maxg) != hipSuccess) { delete h; return fail(PQMC_EHIP, "occupancy query failed"); }
       if ((h->npla + bph - 1) / bph + 1 <= maxg) h->bph = bph;

@@ -12,14 +12,14 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 from pqmc_allinone_amd.host import Chain, Ran2, sweep_batch  # noqa: E402
 from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs  # noqa: E402
 
-K, L, Lt, nsw = (int(v) for v in sys.argv[1:5])
+K, L, Lt, nsw, bph = (int(v) for v in sys.argv[1:6])
 m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=10, ne=L * L))
 x, sl = synthetic_inputs(m, 7)
 nd = 2 * m.Nb * Lt
 
 
 def fresh():
-    ch = Chain(m, helper_bonds=2)
+    ch = Chain(m, helper_bonds=bph)
     ch.set_fields(x)
     ch.set_trial(sl)
     ch.init_g(1)

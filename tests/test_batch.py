@@ -71,3 +71,35 @@ def test_batch_that_does_not_fit_is_refused():
         sweep_batch([chains[0], chains[0]], [False] * 2, [u] * 2)
     for ch in chains:
         ch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bph", [2, 4])
+def test_helper_bonds_match_oracle(bph):
+    """A family pass whose column-owning workgroups hold 2 or 4 bonds each:
+    the same accepts, fields and G as the oracle over alternating sweeps,
+    with decision blocks of 8 and of 3 bonds (blocks that straddle helpers)."""
+    for L, Lt, ng, nb in [(4, 100, 10, 0), (6, 60, 10, 3)]:
+        m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=L * L))
+        x, sl = synthetic_inputs(m, 31)
+        ch = Chain(m, block_bonds=nb, helper_bonds=bph)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        o = O.Oracle(L, L, Lt, ng, L * L)
+        o.set_fields(x)
+        o.set_trial(sl)
+        o.init(1)
+        rng = Ran2(31)
+        fw = True
+        for sweep in range(3):
+            fw = not fw
+            u = rng.draws(2 * m.Nb * Lt)
+            print(f"bph={bph} L={L} nb={nb} sweep {sweep}", flush=True)
+            ch.sweep(fw, u)
+            acc_o, _ = o.sweep(fw, u)
+            assert np.array_equal(ch.accepts(), acc_o), (L, sweep)
+            assert rel(ch.g_h, o.g) < 1e-10, (L, sweep)
+        assert np.array_equal(ch.get_fields(), o.get_fields())
+        assert ch.counters()["sync_timeouts"] == 0
+        ch.close()
