@@ -255,20 +255,35 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
                          sZi[0][l][i] + sZi[1][l][i] + sZi[2][l][i] + sZi[3][l][i]);
   }
   __syncthreads();
-  // zlarft: T(0:i, i) = -tau_i T(0:i, 0:i) z(0:i, i), T(i, i) = tau_i; wave 0, lane k
-  if (wv == 0) {
-    for (int i = 0; i < nb; ++i) {
-      if (lane < i) {
-        zd s = zmk(0.0, 0.0);
-        for (int l = lane; l < i; ++l) s = zadd(s, zmul(sT[lane + 16 * l], sZ[l + 16 * i]));
-        sT[lane + 16 * i] = zmul(zneg(sTaus[i]), s);
-      } else if (lane < 16) {
-        sT[lane + 16 * i] = (lane == i) ? sTaus[i] : zmk(0.0, 0.0);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // T of zlarft (T(0:i, i) = -tau_i T(0:i, 0:i) z(0:i, i), T(i, i) = tau_i)
+  // in closed form: T^-1 = diag(1/tau) + striu(Z), so T = (I + N)^-1 diag(tau)
+  // with N = diag(tau) striu(Z) strictly upper, N^16 = 0, and
+  // (I + N)^-1 = (I - N)(I + N^2)(I + N^4)(I + N^8): six 16 x 16 products,
+  // one entry per thread, instead of 16 dependent steps (and no 1/tau, so
+  // tau = 0 needs no care).  The partial-Gram scratch holds N's powers and W.
+  {
+    zd *sPw = reinterpret_cast<zd *>(&sZr[0][0][0]);  // N, then its powers
+    zd *sWk = sPw + 256;                              // the product so far
+    const int i = tid & 15, k = tid >> 4;
+    const zd ti = i < nb ? sTaus[i] : zmk(0.0, 0.0);
+    const zd nik = (i < k && k < nb) ? zmul(ti, sZ[i + 16 * k]) : zmk(0.0, 0.0);
+    sPw[i + 16 * k] = nik;
+    sWk[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
+    __syncthreads();
+    for (int s = 0; s < 3; ++s) {
+      // P <- P P (N^2, N^4, N^8), then W <- W + W P
+      zd pp = zmk(0.0, 0.0);
+      for (int l = 0; l < 16; ++l) pp = zadd(pp, zmul(sPw[i + 16 * l], sPw[l + 16 * k]));
+      __syncthreads();
+      sPw[i + 16 * k] = pp;
+      __syncthreads();
+      zd wp = zmk(0.0, 0.0);
+      for (int l = 0; l < 16; ++l) wp = zadd(wp, zmul(sWk[i + 16 * l], sPw[l + 16 * k]));
+      __syncthreads();
+      sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
+      __syncthreads();
     }
+    sT[i + 16 * k] = k < nb ? zmul(sWk[i + 16 * k], sTaus[k]) : zmk(0.0, 0.0);
   }
   __syncthreads();
   for (int e = tid; e < nb * nb; e += QR_THREADS) {
