@@ -106,7 +106,9 @@ extern "C" hipError_t pqmc_zgemm(int opA, int opB, int M, int N, int K, zd alpha
 // w = tau^* v^H A, and every thread updates one row.  Three barriers per
 // column, no serial thread-0 loops.
 
-#define QR_THREADS 256
+#define QR_THREADS 1024
+#define QR_WAVES (QR_THREADS / 64)
+#define QR_CPW (16 / QR_WAVES)  // panel columns per wave in the column products
 #define QR_MAXR 16  // rows per lane per load batch: one batch covers mp <= 1024
 
 __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V,
@@ -136,33 +138,35 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
   __syncthreads();
   for (int jj = 0; jj < nb; ++jj) {
     // phase A: wave wv forms the products of column jj (rows > jj) with
-    // columns jj + wv + 4q; for q = 0 on wave 0 that is |x|^2.  Rows come in
+    // columns jj + wv + QR_WAVES q; for q = 0 on wave 0 that is |x|^2.  Rows come in
     // batches of four per lane with every load issued before the products.
     {
-      double pr[4] = {0.0, 0.0, 0.0, 0.0}, pim[4] = {0.0, 0.0, 0.0, 0.0};
+      double pr[QR_CPW], pim[QR_CPW];
+#pragma unroll
+      for (int q = 0; q < QR_CPW; ++q) pr[q] = pim[q] = 0.0;
       for (int r0 = jj + 1; r0 < mp; r0 += 64 * 4) {
-        zd xs[4], as[4][4];
+        zd xs[4], as[QR_CPW][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int r = r0 + lane + 64 * i;
           xs[i] = (r < mp) ? sP[r + mp * jj] : zmk(0.0, 0.0);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int c = jj + wv + 4 * q;
+          for (int q = 0; q < QR_CPW; ++q) {
+            const int c = jj + wv + QR_WAVES * q;
             as[q][i] = (r < mp && c < nb) ? sP[r + mp * c] : zmk(0.0, 0.0);
           }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < QR_CPW; ++q) {
             pr[q] += xs[i].x * as[q][i].x + xs[i].y * as[q][i].y;  // conj(x) a
             pim[q] += xs[i].x * as[q][i].y - xs[i].y * as[q][i].x;
           }
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int c = jj + wv + 4 * q;
+      for (int q = 0; q < QR_CPW; ++q) {
+        const int c = jj + wv + QR_WAVES * q;
         if (c < nb) {  // uniform per wave
           const double sr = wave_sum(pr[q]), si = wave_sum(pim[q]);
           if (lane == 0) {
@@ -227,8 +231,8 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
     sP[e] = v;
   }
   __syncthreads();
-  // Gram V^H V on MFMA: wave wv takes row chunks 4 (wv + 4q)
-  {
+  // Gram V^H V on MFMA: wave wv < 4 takes row chunks 4 (wv + 4q)
+  if (wv < 4) {
     const int li = lane & 15, lk = lane >> 4;
     double4_t zr = {0.0, 0.0, 0.0, 0.0}, zi = {0.0, 0.0, 0.0, 0.0};
     for (int r0 = 4 * wv; r0 < mp; r0 += 16) {
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
   }
   __syncthreads();
   // combine the four partial Gram tiles: z(l, i) into sZ
-  {
+  if (tid < 256) {
     const int l = tid >> 4, i = tid & 15;
     sZ[l + 16 * i] = zmk(sZr[0][l][i] + sZr[1][l][i] + sZr[2][l][i] + sZr[3][l][i],
                          sZi[0][l][i] + sZi[1][l][i] + sZi[2][l][i] + sZi[3][l][i]);
@@ -264,26 +268,31 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, 
   {
     zd *sPw = reinterpret_cast<zd *>(&sZr[0][0][0]);  // N, then its powers
     zd *sWk = sPw + 256;                              // the product so far
-    const int i = tid & 15, k = tid >> 4;
-    const zd ti = i < nb ? sTaus[i] : zmk(0.0, 0.0);
-    const zd nik = (i < k && k < nb) ? zmul(ti, sZ[i + 16 * k]) : zmk(0.0, 0.0);
-    sPw[i + 16 * k] = nik;
-    sWk[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
+    const int i = tid & 15, k = (tid >> 4) & 15;
+    const bool own = tid < 256;  // one entry of the 16 x 16 products per thread
+    if (own) {
+      const zd ti = i < nb ? sTaus[i] : zmk(0.0, 0.0);
+      const zd nik = (i < k && k < nb) ? zmul(ti, sZ[i + 16 * k]) : zmk(0.0, 0.0);
+      sPw[i + 16 * k] = nik;
+      sWk[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
+    }
     __syncthreads();
     for (int s = 0; s < 3; ++s) {
       // P <- P P (N^2, N^4, N^8), then W <- W + W P
       zd pp = zmk(0.0, 0.0);
-      for (int l = 0; l < 16; ++l) pp = zadd(pp, zmul(sPw[i + 16 * l], sPw[l + 16 * k]));
+      if (own)
+        for (int l = 0; l < 16; ++l) pp = zadd(pp, zmul(sPw[i + 16 * l], sPw[l + 16 * k]));
       __syncthreads();
-      sPw[i + 16 * k] = pp;
+      if (own) sPw[i + 16 * k] = pp;
       __syncthreads();
       zd wp = zmk(0.0, 0.0);
-      for (int l = 0; l < 16; ++l) wp = zadd(wp, zmul(sWk[i + 16 * l], sPw[l + 16 * k]));
+      if (own)
+        for (int l = 0; l < 16; ++l) wp = zadd(wp, zmul(sWk[i + 16 * l], sPw[l + 16 * k]));
       __syncthreads();
-      sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
+      if (own) sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
       __syncthreads();
     }
-    sT[i + 16 * k] = k < nb ? zmul(sWk[i + 16 * k], sTaus[k]) : zmk(0.0, 0.0);
+    if (own) sT[i + 16 * k] = k < nb ? zmul(sWk[i + 16 * k], sTaus[k]) : zmk(0.0, 0.0);
   }
   __syncthreads();
   for (int e = tid; e < nb * nb; e += QR_THREADS) {
@@ -316,22 +325,25 @@ extern "C" hipError_t pqmc_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V
 // (v_mfma_f64_16x16x4f64, complex as four real products):
 //   W = V^H A_blk   (nb x 16, the K = mp rows split over the 4 waves),
 //   W2 = op(T) W,   A_blk -= V W2 (16-row tiles over the waves).
-__global__ __launch_bounds__(256) void k_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int ldv,
-                                                    const zd *T, int nb, int trans) {
-  __shared__ double sWr[4][16][17], sWi[4][16][17];
+#define RF_THREADS 256
+#define RF_WAVES (RF_THREADS / 64)
+
+__global__ __launch_bounds__(RF_THREADS) void k_apply_refl(zd *A, int lda, int mp, int ncols, const zd *V, int ldv,
+                                                           const zd *T, int nb, int trans) {
+  __shared__ double sWr[RF_WAVES][16][17], sWi[RF_WAVES][16][17];
   __shared__ double sW2r[16][17], sW2i[16][17];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c0 = blockIdx.x * 16;
   const int li = lane & 15, lk = lane >> 4;
   const int acol = c0 + li;
   const bool acol_ok = acol < ncols;
-  // ---- W = V^H A_blk: wave wv takes row chunks 4 (wv + 4q), 16 chunks per batch ----
+  // ---- W = V^H A_blk: wave wv takes row chunks 4 (wv + RF_WAVES q), 16 chunks per batch ----
   double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-  for (int r0 = 4 * wv; r0 < mp; r0 += 16 * 16) {
+  for (int r0 = 4 * wv; r0 < mp; r0 += 4 * RF_WAVES * 16) {
     zd vv[16], xx[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const int r = r0 + 16 * u + lk;  // chunk r0 + 16u, row lk of it
+      const int r = r0 + 4 * RF_WAVES * u + lk;  // chunk r0 + 4 RF_WAVES u, row lk of it
       vv[u] = (r < mp && li < nb) ? V[r + (size_t)ldv * li] : zmk(0.0, 0.0);
       xx[u] = (r < mp && acol_ok) ? A[r + (size_t)lda * acol] : zmk(0.0, 0.0);
     }
@@ -351,14 +363,18 @@ __global__ __launch_bounds__(256) void k_apply_refl(zd *A, int lda, int mp, int 
   }
   __syncthreads();
   // ---- W2 = op(T) W ----
-  {
+  if (tid < 256) {
     const int kk = tid >> 4, cc = tid & 15;
     double wr = 0.0, wi = 0.0;
     if (kk < nb) {
       for (int k = 0; k < nb; ++k) {
         const zd tk = trans ? zconj(T[k + nb * kk]) : T[kk + nb * k];
-        const double xr = sWr[0][k][cc] + sWr[1][k][cc] + sWr[2][k][cc] + sWr[3][k][cc];
-        const double xi = sWi[0][k][cc] + sWi[1][k][cc] + sWi[2][k][cc] + sWi[3][k][cc];
+        double xr = 0.0, xi = 0.0;
+#pragma unroll
+        for (int w = 0; w < RF_WAVES; ++w) {
+          xr += sWr[w][k][cc];
+          xi += sWi[w][k][cc];
+        }
         wr += tk.x * xr - tk.y * xi;
         wi += tk.x * xi + tk.y * xr;
       }
@@ -374,12 +390,12 @@ __global__ __launch_bounds__(256) void k_apply_refl(zd *A, int lda, int mp, int 
     w2r[k4] = sW2r[4 * k4 + lk][li];
     w2i[k4] = sW2i[4 * k4 + lk][li];
   }
-  // 16-row tiles t0 = 16 (wv + 4q), four tiles per batch: V and A loads first
-  for (int tb = 16 * wv; tb < mp; tb += 4 * 64) {
+  // 16-row tiles t0 = 16 (wv + RF_WAVES q), four tiles per batch: V and A loads first
+  for (int tb = 16 * wv; tb < mp; tb += 4 * 16 * RF_WAVES) {
     zd vv[4][4], aa[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int t0 = tb + 64 * q;
+      const int t0 = tb + 16 * RF_WAVES * q;
       const int r = t0 + li;
 #pragma unroll
       for (int k4 = 0; k4 < 4; ++k4) {
@@ -394,7 +410,7 @@ __global__ __launch_bounds__(256) void k_apply_refl(zd *A, int lda, int mp, int 
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int t0 = tb + 64 * q;
+      const int t0 = tb + 16 * RF_WAVES * q;
       double4_t pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int k4 = 0; k4 < 4; ++k4) {
@@ -417,8 +433,8 @@ extern "C" hipError_t pqmc_apply_refl(zd *A, int lda, int mp, int ncols, const z
                                       int nb, int trans, hipStream_t s) {
   if (ncols <= 0) return hipSuccess;
   if (nb > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_apply_refl, dim3((ncols + 15) / 16), dim3(256), 0, s, A, lda, mp, ncols, V, ldv, T, nb,
-                     trans);
+  hipLaunchKernelGGL(k_apply_refl, dim3((ncols + 15) / 16), dim3(RF_THREADS), 0, s, A, lda, mp, ncols, V, ldv, T,
+                     nb, trans);
   return hipGetLastError();
 }
 
