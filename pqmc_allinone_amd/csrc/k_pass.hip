@@ -911,6 +911,52 @@ static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh
   }
 }
 
+// The same update on FP64 MFMA when a helper owns 16 columns (four bonds):
+// per 16-row tile, C(16 x 16) += P(16 x n4) Y(n4 x 16) with P the pivot
+// columns G_k[:, S_A] and Y[g][q] = y[q][g], complex as four real products.
+// Lane l supplies P[t0 + l%16][g0 + l/16] and Y[g0 + l/16][l%16] and holds C
+// rows l/16 + 4i of column l%16.  All pivot loads of a tile are issued
+// before the first product.
+static __device__ void ps_pivot_update_mfma(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4,
+                                            const zd *sy, zd *sC) {
+  const int Ns = A.md.Ns, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr int NWV = PS_THREADS / 64, NCH = PS_BS / 4, NT = 4;  // NT tiles per wave cover Ns <= 512
+  zd p[NT][NCH];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int r = 16 * (wv + NWV * j) + li;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      const int g = 4 * ch + lk;
+      p[j][ch] = (4 * ch < n4 && r < Ns) ? ps_scol(A, kstep, 0, 4 * Hh.li[g >> 2] + (g & 3))[r] : zmk(0.0, 0.0);
+    }
+  }
+  zd y[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) y[ch] = 4 * ch < n4 ? sy[li * PS_BS + 4 * ch + lk] : zmk(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int t0 = 16 * (wv + NWV * j);
+    if (t0 >= Ns) break;  // uniform
+    double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      if (4 * ch < n4) {  // uniform
+        cr = mfma_f64(p[j][ch].x, y[ch].x, cr);
+        cr = mfma_f64(-p[j][ch].y, y[ch].y, cr);
+        ci = mfma_f64(p[j][ch].x, y[ch].y, ci);
+        ci = mfma_f64(p[j][ch].y, y[ch].x, ci);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = t0 + lk + 4 * i;
+      if (row < Ns) sC[li * Ns + row] = zadd(sC[li * Ns + row], zmk(cr[i], ci[i]));
+    }
+  }
+}
+
 template <int BPH>
 static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy, int *sS) {
   constexpr int NC = 4 * BPH;
@@ -1017,7 +1063,10 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
       __syncthreads();
       PROF_ACC(3);
       // G[:, q] += G_k[:, S_A] y[q]; pivot columns from the slot of step k-1
-      ps_pivot_update(A, Hh, k - 1, n4, NC, sy, sC);
+      if constexpr (BPH == 4)
+        ps_pivot_update_mfma(A, Hh, k - 1, n4, sy, sC);
+      else
+        ps_pivot_update(A, Hh, k - 1, n4, NC, sy, sC);
       __syncthreads();
     }
     PROF_ACC(5);
@@ -1088,6 +1137,7 @@ extern "C" size_t pqmc_pass_rec_bytes(void) { return sizeof(ps_rec); }
 
 extern "C" hipError_t pqmc_launch_pass(const ps_args *a, hipStream_t s) {
   if (a->bph != 1 && a->bph != 2 && a->bph != 4) return hipErrorInvalidValue;
+  if (a->bph == 4 && a->md.Ns > 512) return hipErrorInvalidValue;  // ps_pivot_update_mfma covers 512 rows
   const size_t lds = pqmc_pass_lds(a->md.Ns, a->bph);
   static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 0;
