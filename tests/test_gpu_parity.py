@@ -147,10 +147,11 @@ def test_sweeps_match_oracle(nb):
     assert (c["positive_accept"], c["negative_accept"]) == (pa, na)
 
 
-@pytest.mark.parametrize("L,Lt,ntc,nb", [(8, 40, 20, 0), (8, 40, 20, 5), (14, 40, 10, 0)])
+@pytest.mark.parametrize("L,Lt,ntc,nb", [(8, 40, 20, 0), (8, 40, 20, 5), (14, 40, 10, 0), (20, 40, 20, 0)])
 def test_partial_sweeps_match_oracle_larger_lattices(L, Lt, ntc, nb):
-    """8x8 (4 blocks of 8 bonds, or 7 blocks of 5 with a ragged last block) and
-    14x14 (the production size, 13 blocks): identical accepts and G after
+    """8x8 (4 blocks of 8 bonds, or 7 blocks of 5 with a ragged last block),
+    14x14 (the production size, 13 blocks) and 20x20 (Ns = 800, ne = 400: 201
+    workgroups per pass, 8-column QR panels): identical accepts and G after
     partial sweeps in both directions, with stabilization every 10 slices."""
     m, orc, ch = make_pair(L=L, Lt=Lt, ngroup=10, ne=L * L, seed=7, nb=nb)
     rng = O.Ran2Stream(7)
