@@ -128,6 +128,17 @@ int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, int measure
 int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forward, const float *const *u, int measure,
                      double *const *meas_out);
 
+/* Any number of chains, one sweep each (the batched form for more chains
+ * than fit together, e.g. 128 chains of uneven lattice and Lt): up to
+ * max_concurrent host threads (0 = 16) take the chains in order, and a
+ * chain's sweep starts once the compute units its family passes need fit
+ * beside those of the sweeps already running.  A chain that needs more
+ * compute units than the device has is refused with PQMC_EINVAL before
+ * anything runs.  Returns the first failing chain's status.  No reference
+ * counterpart (src/Main_PQMC.f90:380-400 runs one chain per MPI rank). */
+int pqmc_sweep_many(pqmc_handle *const *hs, int n, const int *forward, const float *const *u, int measure,
+                    double *const *meas_out, int max_concurrent);
+
 /* State readers/writers (parity and checkpointing). */
 int pqmc_get_g(pqmc_handle *h, double *g);
 int pqmc_set_g(pqmc_handle *h, const double *g);

@@ -58,6 +58,7 @@ SIGNATURES = {
     "pqmc_measure": (_I, [_P, _I, _P]),
     "pqmc_sweep": (_I, [_P, _I, _P, _I, _I, _P]),
     "pqmc_sweep_batch": (_I, [_P, _I, _P, _P, _I, _P]),
+    "pqmc_sweep_many": (_I, [_P, _I, _P, _P, _I, _P, _I]),
     "pqmc_get_g": (_I, [_P, _P]),
     "pqmc_set_g": (_I, [_P, _P]),
     "pqmc_get_qstring": (_I, [_P, _I, _P]),

@@ -31,9 +31,11 @@ template <int N>
 PQ_DEV zd wick_contract(const meas_ctx &c, const int (&ls)[N][2], const int (&rs)[N][2], const zd *vl,
                         const zd *vr) {
   zd tot = zmk(0.0, 0.0);
+#pragma unroll
   for (int i = 0; i < N; ++i) {
     zd t = zmk(0.0, 0.0);
     const zd hl = c.gh(ls[i][1], ls[i][0]);
+#pragma unroll
     for (int j = 0; j < N; ++j) {
       zd m = zmul(hl, c.gh(rs[j][1], rs[j][0]));
       m = zadd(m, zmul(c.gh(rs[j][1], ls[i][0]), c.g(ls[i][1], rs[j][0])));
@@ -82,7 +84,27 @@ PQ_DEV int cell_of(int x, int y, int L1, int L2) {
 
 #define MEAS_THREADS 256
 
+// the launch (GROUP of k_meas_pairs) that computes field observable q
+// (Jxx_Jyy and Jxy_Jyx are registered but never recorded: group 0 writes
+// their zeros)
+PQ_DEV int meas_group(int q) {
+  if (q <= F_JxyJxy || q == F_JxxJyy || q == F_JxyJyx) return 0;
+  if (q == F_JyxJyx || q == F_JyyJyy) return 4;
+  if (q == F_JxxJyx || q == F_JxyJyy) return 5;
+  if (q == F_cJyx) return 2;
+  if (q == F_cJxy) return 3;
+  return 1;
+}
+
 // one workgroup per displacement d; the pair (c1, c1 + d) has relative index d
+// The observables are split over six launches (GROUP 0: bosons and the
+// x-bond spin currents, 4: y-bond, 5: x-bond against y-bond, 1: spin,
+// density and pairing, 2 and 3: the two 8 x 8 physical currents), so that no
+// launch holds more G values in registers than fit: one launch for all of
+// them spilled 4 KB per lane to scratch, which every hardware queue running
+// it must back.  Each observable keeps its own per-thread accumulation and
+// reduction order.
+template <int GROUP>
 __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const zd *G, const double *x, int time,
                                                              zd *out) {
   const int L1 = md.L1, L2 = md.L2, Nc = md.Ncell;
@@ -103,12 +125,13 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
     const int c2x = cell_of(x2 + 1, y2, L1, L2), c2y = cell_of(x2, y2 + 1, L1, L2);
     const bool cb1x = (x1 == L1 - 1), cb1y = (y1 == L2 - 1), cb2x = (x2 == L1 - 1), cb2y = (y2 == L2 - 1);
     const double b1x = xt[2 * c1], b1y = xt[2 * c1 + 1], b2x = xt[2 * c2], b2y = xt[2 * c2 + 1];
+    int ls[4][2], rs[4][2];
+    zd vl[8], vr[8];
+    if constexpr (GROUP == 0) {
     acc[F_BFxBFx] = zadd(acc[F_BFxBFx], zmk(b1x * b2x / Nc, 0.0));
     acc[F_BFxBFy] = zadd(acc[F_BFxBFy], zmk(b1x * b2y / Nc, 0.0));
     acc[F_BFyBFy] = zadd(acc[F_BFyBFy], zmk(b1y * b2y / Nc, 0.0));
     acc[F_BFxyBFxy] = zadd(acc[F_BFxyBFxy], zmk(b1x * b1y * b2x * b2y / Nc, 0.0));
-    int ls[4][2], rs[4][2];
-    zd vl[8], vr[8];
     // x-bond spin currents
     spin_current_lists(c1, c1x, ls);
     spin_current_lists(c2, c2x, rs);
@@ -118,6 +141,8 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
     for (int q = 0; q < 4; ++q) { vl[q] = vjy[q]; vr[q] = vjy[q]; }
     bphase(vl, cb1x, md.bc[0]); bphase(vr, cb2x, md.bc[0]);
     acc[F_JxyJxy] = zadd(acc[F_JxyJxy], zscale(wick_contract<4>(mc, ls, rs, vl, vr), inv));
+    }
+    if constexpr (GROUP == 4) {
     // y-bond spin currents
     spin_current_lists(c1, c1y, ls);
     spin_current_lists(c2, c2y, rs);
@@ -128,6 +153,8 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
     bphase(vl, cb1y, md.bc[1]); bphase(vr, cb2y, md.bc[1]);
     // Jyy_Jyy contracts with vec_l on both sides (src/mod_meas.f90:353)
     acc[F_JyyJyy] = zadd(acc[F_JyyJyy], zscale(wick_contract<4>(mc, ls, rs, vl, vl), inv));
+    }
+    if constexpr (GROUP == 5) {
     // x-bond vs y-bond
     spin_current_lists(c1, c1x, ls);
     spin_current_lists(c2, c2y, rs);
@@ -137,6 +164,8 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
     for (int q = 0; q < 4; ++q) { vl[q] = vjy[q]; vr[q] = vjy[q]; }
     bphase(vl, cb1x, md.bc[0]); bphase(vr, cb2y, md.bc[1]);
     acc[F_JxyJyy] = zadd(acc[F_JxyJyy], zscale(wick_contract<4>(mc, ls, rs, vl, vr), inv));
+    }
+    if constexpr (GROUP == 1) {
     // spin momentum
     spin_momentum_lists(c1, ls);
     spin_momentum_lists(c2, rs);
@@ -157,8 +186,9 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
       zd y[4] = {zmk(0, 0), zmk(0, -0.5), zmk(0, 0.5), zmk(0, 0)};
       acc[F_Sy] = zadd(acc[F_Sy], zscale(wick_contract<4>(mc, ls, rs, y, y), inv));
     }
+    }
     // physical currents (8x8): y-bond spin-x with kinetic part, x-bond spin-y
-    {
+    if constexpr (GROUP == 2) {
       int l8[8][2], r8[8][2];
       phys_lists(c1, c1y, l8);
       phys_lists(c2, c2y, r8);
@@ -169,6 +199,9 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
       bphase(vl, cb1y, md.bc[1]); bphase(vr, cb2y, md.bc[1]);
       bphase(vl + 4, cb1y, md.bc[1]); bphase(vr + 4, cb2y, md.bc[1]);
       acc[F_cJyx] = zadd(acc[F_cJyx], zscale(wick_contract<8>(mc, l8, r8, vl, vr), inv));
+    }
+    if constexpr (GROUP == 3) {
+      int l8[8][2], r8[8][2];
       phys_lists(c1, c1x, l8);
       phys_lists(c2, c2x, r8);
       const double sy[4] = {-1.0, 1.0, 1.0, -1.0};
@@ -188,7 +221,7 @@ __global__ __launch_bounds__(MEAS_THREADS) void k_meas_pairs(model_t md, const z
     if (lane == 0) { sr[wv][q] = vr_; si[wv][q] = vi_; }
   }
   __syncthreads();
-  if (threadIdx.x < N_FOBS) {
+  if (threadIdx.x < N_FOBS && meas_group(threadIdx.x) == GROUP) {
     const int q = threadIdx.x;
     double a = 0.0, b = 0.0;
     for (int w = 0; w < MEAS_THREADS / 64; ++w) { a += sr[w][q]; b += si[w][q]; }
@@ -233,7 +266,12 @@ __global__ void k_meas_kspace(int Nc, const zd *phase /*Nc x Nc, (k, i)*/, zd *o
 
 extern "C" hipError_t pqmc_launch_measure(const model_t *md, const zd *G, const double *x, int time,
                                           const zd *phase, zd *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_meas_pairs, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<0>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<1>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<2>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<3>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<4>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
+  hipLaunchKernelGGL(k_meas_pairs<5>, dim3(md->Ncell), dim3(MEAS_THREADS), 0, s, *md, G, x, time, out);
   hipLaunchKernelGGL(k_meas_scalars, dim3(1), dim3(256), 0, s, *md, x, time, out);
   hipLaunchKernelGGL(k_meas_kspace, dim3((md->Ncell + 63) / 64, N_FOBS), dim3(64), 0, s, md->Ncell, phase, out);
   return hipGetLastError();

@@ -8,6 +8,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -969,14 +973,19 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   return PQMC_OK;
 }
 
-// Several chains, one sweep each, at the same time: one host thread per
-// chain drives pqmc_sweep on the chain's own stream.  A family pass spins
-// until all of its workgroups are resident, so the passes of every chain of
-// the batch must fit the device together; the batch is refused otherwise.
-extern "C" int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forward, const float *const *u,
-                                int measure, double *const *meas_out) {
-  if (!hs || !forward || !u || n < 1) return fail(PQMC_EINVAL, "null argument or empty batch");
-  int ncu = 0;
+// Compute units one chain's family passes keep resident: every workgroup of
+// a pass must be resident at once, and each holds the leader's LDS.
+static int pass_cus(pqmc_handle *h, int ncu, int *need) {
+  int maxg = 0;
+  HIPCHK(pqmc_pass_max_groups(h->Ns, h->bph, &maxg));
+  const int per_cu = maxg / (ncu > 0 ? ncu : 1);
+  if (per_cu < 1) return fail(PQMC_EINVAL, "a family pass workgroup does not fit one compute unit");
+  *need = (h->H + 1 + per_cu - 1) / per_cu;
+  return PQMC_OK;
+}
+
+static int check_chains(pqmc_handle *const *hs, int n, const float *const *u, int *ncu) {
+  if (!hs || !u || n < 1) return fail(PQMC_EINVAL, "null argument or empty batch");
   for (int i = 0; i < n; ++i) {
     if (!hs[i] || !u[i]) return fail(PQMC_EINVAL, "null handle or draws in the batch");
     for (int j = 0; j < i; ++j)
@@ -984,14 +993,26 @@ extern "C" int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forwar
     if (hs[i]->dev != hs[0]->dev) return fail(PQMC_EINVAL, "the chains of a batch must share one device");
   }
   HIPCHK(hipSetDevice(hs[0]->dev));
-  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hs[0]->dev));
+  HIPCHK(hipDeviceGetAttribute(ncu, hipDeviceAttributeMultiprocessorCount, hs[0]->dev));
+  return PQMC_OK;
+}
+
+// Several chains, one sweep each, at the same time: one host thread per
+// chain drives pqmc_sweep on the chain's own stream.  A family pass spins
+// until all of its workgroups are resident, so the passes of every chain of
+// the batch must fit the device together; the batch is refused otherwise.
+extern "C" int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forward, const float *const *u,
+                                int measure, double *const *meas_out) {
+  if (!forward) return fail(PQMC_EINVAL, "null argument");
+  int ncu = 0;
+  int rc0 = check_chains(hs, n, u, &ncu);
+  if (rc0) return rc0;
   long need = 0;  // compute units the passes occupy together
   for (int i = 0; i < n; ++i) {
-    int maxg = 0;
-    HIPCHK(pqmc_pass_max_groups(hs[i]->Ns, hs[i]->bph, &maxg));
-    const int per_cu = maxg / (ncu > 0 ? ncu : 1);
-    if (per_cu < 1) return fail(PQMC_EINVAL, "a family pass workgroup does not fit one compute unit");
-    need += (hs[i]->H + 1 + per_cu - 1) / per_cu;
+    int c = 0;
+    rc0 = pass_cus(hs[i], ncu, &c);
+    if (rc0) return rc0;
+    need += c;
   }
   if (need > ncu)
     return fail(PQMC_EINVAL, "the batch's family passes (" + std::to_string(need) + " compute units) do not fit the " +
@@ -1014,6 +1035,68 @@ extern "C" int pqmc_sweep_batch(pqmc_handle *const *hs, int n, const int *forwar
   for (auto &t : th) t.join();
   for (int i = 0; i < n; ++i)
     if (rc[i]) return fail(rc[i], "chain " + std::to_string(i) + " of the batch: " + msg[i]);
+  return PQMC_OK;
+}
+
+// Any number of chains, one sweep each.  Up to max_concurrent host threads
+// take the chains in order; a thread starts a chain's sweep once the compute
+// units its family passes need fit beside those of the sweeps already
+// running (a budget of the device's compute units), and returns them when
+// the sweep ends.  So the passes in flight always fit together, whatever
+// the mix of lattices; a chain that needs more than the device is refused
+// before anything runs.
+extern "C" int pqmc_sweep_many(pqmc_handle *const *hs, int n, const int *forward, const float *const *u,
+                               int measure, double *const *meas_out, int max_concurrent) {
+  if (!forward) return fail(PQMC_EINVAL, "null argument");
+  int ncu = 0;
+  int rc0 = check_chains(hs, n, u, &ncu);
+  if (rc0) return rc0;
+  std::vector<int> need(n);
+  for (int i = 0; i < n; ++i) {
+    rc0 = pass_cus(hs[i], ncu, &need[i]);
+    if (rc0) return rc0;
+    if (need[i] > ncu)
+      return fail(PQMC_EINVAL, "chain " + std::to_string(i) + " needs " + std::to_string(need[i]) +
+                                   " compute units for its family passes, more than the device has");
+  }
+  const int nthr = std::max(1, std::min(n, max_concurrent > 0 ? max_concurrent : 16));
+  std::mutex mu;
+  std::condition_variable cv;
+  int free_cus = ncu;
+  std::atomic<int> next{0};
+  std::vector<int> rc(n, PQMC_OK);
+  std::vector<std::string> msg(n);
+  auto worker = [&]() {
+    if (hipSetDevice(hs[0]->dev) != hipSuccess) {
+      std::lock_guard<std::mutex> lk(mu);
+      for (int i = next.exchange(n); i < n; ++i) {
+        rc[i] = PQMC_EHIP;
+        msg[i] = "hipSetDevice failed";
+      }
+      return;
+    }
+    for (int i = next++; i < n; i = next++) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return free_cus >= need[i]; });
+        free_cus -= need[i];
+      }
+      rc[i] = pqmc_sweep(hs[i], forward[i], u[i], 0, measure, meas_out ? meas_out[i] : nullptr);
+      if (rc[i]) msg[i] = g_err;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        free_cus += need[i];
+      }
+      cv.notify_all();
+    }
+  };
+  std::vector<std::thread> th;
+  th.reserve(nthr - 1);
+  for (int t = 1; t < nthr; ++t) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rc[i]) return fail(rc[i], "chain " + std::to_string(i) + ": " + msg[i]);
   return PQMC_OK;
 }
 

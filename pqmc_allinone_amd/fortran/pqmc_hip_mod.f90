@@ -132,6 +132,15 @@ module pqmc_hip
       type(c_ptr), value :: meas_out
     end function pqmc_sweep_batch
 
+    integer(c_int) function pqmc_sweep_many(hs, n, forward, u, measure, meas_out, max_concurrent) &
+        bind(C, name='pqmc_sweep_many')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: hs(*), u(*)
+      integer(c_int), value :: n, measure, max_concurrent
+      integer(c_int), intent(in) :: forward(*)
+      type(c_ptr), value :: meas_out
+    end function pqmc_sweep_many
+
     integer(c_int) function pqmc_get_g(h, g) bind(C, name='pqmc_get_g')
       import :: c_int, c_ptr, c_double_complex
       type(c_ptr), value :: h

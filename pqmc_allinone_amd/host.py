@@ -366,10 +366,13 @@ def read_driver_output(path: str, model: Model, nsweeps: int) -> dict:
             "global_accept": int(cnt[2]), "global_reject": int(cnt[3])}
 
 
-def sweep_batch(chains, forwards, draws, measure: bool = False):
+def sweep_batch(chains, forwards, draws, measure: bool = False, many: bool = False, max_concurrent: int = 0):
     """One sweep of every chain, at the same time (pqmc_sweep_batch): one
     host thread per chain, each on its chain's stream.  Returns the
-    measurements (one total_width vector per chain) when measure is set."""
+    measurements (one total_width vector per chain) when measure is set.
+    many: any number of chains (pqmc_sweep_many), each started as soon as
+    its family passes fit the device beside the sweeps already running,
+    with up to max_concurrent host threads (0: 16)."""
     n = len(chains)
     if n < 1 or len(forwards) != n or len(draws) != n:
         raise ValueError("one direction and one draw array per chain")
@@ -384,8 +387,17 @@ def sweep_batch(chains, forwards, draws, measure: bool = False):
     up = (C.c_void_p * n)(*[u.ctypes.data for u in us])
     outs = [np.empty(ch.total_width, np.complex128) for ch in chains] if measure else None
     op = (C.c_void_p * n)(*[o.ctypes.data for o in outs]) if measure else None
-    _lib.check(_lib.lib().pqmc_sweep_batch(hs, n, fw, up, int(bool(measure)), op))
+    if many:
+        _lib.check(_lib.lib().pqmc_sweep_many(hs, n, fw, up, int(bool(measure)), op, int(max_concurrent)))
+    else:
+        _lib.check(_lib.lib().pqmc_sweep_batch(hs, n, fw, up, int(bool(measure)), op))
     return outs
+
+
+def sweep_many(chains, forwards, draws, measure: bool = False, max_concurrent: int = 0):
+    """One sweep of every chain of any number of chains of any sizes
+    (pqmc_sweep_many)."""
+    return sweep_batch(chains, forwards, draws, measure, many=True, max_concurrent=max_concurrent)
 
 
 class GlobalMoves:

@@ -103,3 +103,41 @@ def test_helper_bonds_match_oracle(bph):
         assert np.array_equal(ch.get_fields(), o.get_fields())
         assert ch.counters()["sync_timeouts"] == 0
         ch.close()
+
+
+@pytest.mark.gpu
+def test_many_uneven_chains_match_lone_chains():
+    """pqmc_sweep_many over more chains than fit together (24 chains of
+    L in {4, 6, 8, 10}, Lt in {40, 60, 80}, one or four bonds per helper,
+    8 host threads): each chain ends two sweeps bit for bit where its twin,
+    swept alone, ends."""
+    from pqmc_allinone_amd.host import sweep_many
+    specs = [(L, Lt, bph, seed) for i, (L, Lt) in enumerate([(L, Lt) for L in (4, 6, 8, 10) for Lt in (40, 60, 80)])
+             for bph, seed in [(1, 100 + i), (4, 200 + i)]]
+
+    def make(L, Lt, bph, seed):
+        m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=10, ne=L * L))
+        x, sl = synthetic_inputs(m, seed)
+        ch = Chain(m, helper_bonds=bph)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        return m, ch
+
+    pairs = [(make(*sp), make(*sp)) for sp in specs]
+    rngs = [Ran2(sp[3]) for sp in specs]
+    fw = True
+    for sweep in range(2):
+        fw = not fw
+        draws = [r.draws(2 * m.Nb * m.p.Lt) for r, ((m, _), _) in zip(rngs, pairs)]
+        print(f"sweep_many: {len(pairs)} chains, sweep {sweep}", flush=True)
+        outs = sweep_many([a for (_, a), _ in pairs], [fw] * len(pairs), draws, measure=True, max_concurrent=8)
+        for i, ((m, a), (_, b)) in enumerate(pairs):
+            mb = b.sweep(fw, draws[i], measure=True)
+            assert np.array_equal(a.accepts(), b.accepts()), (i, sweep)
+            assert np.array_equal(a.g_h, b.g_h), (i, sweep)
+            assert np.array_equal(outs[i], mb), (i, sweep)
+    for (_, a), (_, b) in pairs:
+        assert a.counters()["sync_timeouts"] == 0
+        a.close()
+        b.close()
