@@ -107,20 +107,23 @@ def test_helper_bonds_match_oracle(bph):
 
 @pytest.mark.gpu
 def test_many_uneven_chains_match_lone_chains():
-    """pqmc_sweep_many over more chains than fit together (24 chains of
-    L in {4, 6, 8, 10}, Lt in {40, 60, 80}, one or four bonds per helper,
-    8 host threads): each chain ends two sweeps bit for bit where its twin,
-    swept alone, ends."""
+    """pqmc_sweep_many over more chains than fit together (24 projector
+    chains of L in {4, 6, 8, 10}, Lt in {40, 60, 80}, one or four bonds per
+    helper, and two finite-temperature chains; 8 host threads): each chain
+    ends two sweeps bit for bit where its twin, swept alone, ends."""
     from pqmc_allinone_amd.host import sweep_many
-    specs = [(L, Lt, bph, seed) for i, (L, Lt) in enumerate([(L, Lt) for L in (4, 6, 8, 10) for Lt in (40, 60, 80)])
+    specs = [(L, Lt, bph, seed, False) for i, (L, Lt) in
+             enumerate([(L, Lt) for L in (4, 6, 8, 10) for Lt in (40, 60, 80)])
              for bph, seed in [(1, 100 + i), (4, 200 + i)]]
+    specs += [(4, 40, 1, 300, True), (6, 60, 4, 301, True)]
 
-    def make(L, Lt, bph, seed):
+    def make(L, Lt, bph, seed, ft):
         m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=10, ne=L * L))
         x, sl = synthetic_inputs(m, seed)
-        ch = Chain(m, helper_bonds=bph)
+        ch = Chain(m, helper_bonds=bph, finite_temperature=ft)
         ch.set_fields(x)
-        ch.set_trial(sl)
+        if not ft:
+            ch.set_trial(sl)
         ch.init_g(1)
         return m, ch
 
