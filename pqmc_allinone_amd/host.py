@@ -15,6 +15,9 @@ Meas_sys%begin_measure       ``measio.MeasurementSystem.begin_measure``
 update_global(kind)          ``GlobalMoves.update(kind, x, ln_cw)``
 loop body (one sweep)        ``Chain.sweep(forward, draws, measure)``
 ran2_s / rands / ran_sysm    ``Ran2`` (src/mod_ranstate.f90:171-236)
+proj = .false. (_FT family)  ``Chain(model, finite_temperature=True)``
+cal_new_weight_T0 / _FT      ``Chain.global_weight(x)``
+one chain per MPI rank       ``sweep_batch`` / ``sweep_many`` (chains together)
 ===========================  ============================================
 
 All state lives on the GPU; ``g_h``, fields and B-string checkpoints are
