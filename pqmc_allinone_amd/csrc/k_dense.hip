@@ -109,7 +109,6 @@ extern "C" hipError_t pqmc_zgemm(int opA, int opB, int M, int N, int K, zd alpha
 #define QR_THREADS 1024
 #define QR_WAVES (QR_THREADS / 64)
 #define QR_CPW (16 / QR_WAVES)  // panel columns per wave in the column products
-#define QR_MAXR 16  // rows per lane per load batch: one batch covers mp <= 1024
 
 __global__ __launch_bounds__(QR_THREADS) void k_qr_panel(zd *A, int lda, int m, int j0, int nb, zd *V,
                                                           int ldv, zd *T, zd *tau, double *vn2,
