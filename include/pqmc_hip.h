@@ -166,6 +166,17 @@ int pqmc_synchronize(pqmc_handle *h);
  * fields and runs pqmc_init_g(1), as src/Main_PQMC.f90:94 does. */
 int pqmc_global_weight(pqmc_handle *h, const double *x, double *re_im);
 
+/* Field gradient of the fermionic log-weight: grad (complex, Nb*Lt,
+ * boson_field layout) = d ln_cw / dx for every field, as Tr[G dE E^-1] along
+ * a backward pass over the slices with no updates, G recomputed from the
+ * checkpointed B-strings every ngroup slices.  No counterpart in the
+ * reference; it is the adjoint a hybrid-Monte-Carlo global move needs
+ * beside update_global (src/mod_update.f90:251-290).  Starts where a
+ * backward pqmc_sweep starts (after pqmc_init_g(1) or a forward sweep) and
+ * leaves the chain where a backward sweep that accepts nothing leaves it.
+ * Projector and finite-temperature chains alike. */
+int pqmc_gradient(pqmc_handle *h, double *grad);
+
 /* The ran2 generator of src/mod_ranstate.f90:171-191 seeded as
  * init_rng_by_seed (:267-283); host-side, for drivers without their own. */
 typedef struct pqmc_ran2 { int32_t i, j, k, m, n; } pqmc_ran2;

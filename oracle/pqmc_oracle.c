@@ -1486,6 +1486,91 @@ void orc_cal_new_weight(orc_model *m, double *ri) {
   free(lmat);
 }
 
+/* ------------------------------------------------------------------ */
+/* field gradient of ln_cw (no counterpart in the reference)           */
+/* ------------------------------------------------------------------ */
+
+/* d/dx of expkv(.., inv = 0), same layout.  With n = delt sqrt(|K|^2 +
+ * (|V| x)^2), s = sinh(n)/n and n dn/dx = delt^2 |V|^2 x =: nd:
+ * d cosh(n)/dx = s nd and ds/dx = r nd, r = (n cosh n - sinh n)/n^3, taken
+ * from its series 1/3 + n^2/30 below n = 1e-3; dh/dx = -delt V_coe V12. */
+static void dexpkv(const orc_model *m, int f, int b, double xv, zc *de) {
+  double ak = cabs(m->kcoe[f]), av = cabs(m->vcoe[f]);
+  double n = m->dtau * sqrt(ak * ak + (av * xv) * (av * xv));
+  double nd = m->dtau * m->dtau * av * av * xv;
+  double s = n > 0 ? sinh(n) / n : 1.0;
+  double r = n < 1e-3 ? 1.0 / 3.0 + n * n / 30.0 : (n * cosh(n) - sinh(n)) / (n * n * n);
+  zc sub[4], dsub[4];
+  zc vx = m->vcoe[f] * xv;
+  for (int q = 0; q < 4; ++q) {
+    sub[q] = -m->dtau * (m->kcoe[f] * m->k12[f][q] + vx * m->v12[f][q]);
+    dsub[q] = -m->dtau * m->vcoe[f] * m->v12[f][q];
+  }
+  if (m->cross[f * m->npla + b]) {
+    const zc *ph = &m->bcph[(f * m->npla + b) * 16];
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) {
+        sub[a + 2 * c] = sub[a + 2 * c] * ph[a + 4 * (2 + c)];
+        dsub[a + 2 * c] = dsub[a + 2 * c] * ph[a + 4 * (2 + c)];
+      }
+  }
+  for (int q = 0; q < 16; ++q) de[q] = 0;
+  for (int i = 0; i < 4; ++i) de[i + 4 * i] = s * nd;
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) {
+      zc v = dsub[a + 2 * c] * s + sub[a + 2 * c] * (r * nd);
+      de[a + 4 * (2 + c)] = v;
+      de[(2 + c) + 4 * a] = conj(v);
+    }
+}
+
+void orc_get_dexpkv(const orc_model *m, int pf_id, int pla, double xv, double *out) {
+  dexpkv(m, pf_id - 1, pla, xv, (zc *)out);
+}
+
+/* d ln_cw / dx(i, t) for every field, complex, grad[i + Nb (t-1)].  A
+ * backward pass over the slices without updates: where E_f(t) is the
+ * outermost factor of G's right string (just before the backward sweep's
+ * two-way wrap E^-1 G E), d ln det(L E R') / dx = Tr[G dE E^-1], and for a
+ * bond only its four sites contribute: sum_ij (de e^-1)(i,j) G(S_j, S_i).
+ * The same holds for g_h = 1 - (1 + B)^-1 and ln det(1 + B) at finite
+ * temperature.  ln_cw's tr_weight doubling and ncopy power are applied.
+ * Starts where a backward sweep starts (after init_g(1) or a forward sweep)
+ * and leaves the chain where a backward sweep that rejects every proposal
+ * leaves it. */
+void orc_gradient(orc_model *m, double *grad_out) {
+  int Lt = m->Lt, npla = m->npla, Ns = m->Ns;
+  size_t Nb = m->Nb;
+  zc *grad = (zc *)grad_out;
+  for (size_t i = 0; i < Nb * Lt; ++i) grad[i] = 0;
+  zc de[16], md[16];
+  for (int tc = 1; tc <= Lt; ++tc) {
+    int time = Lt - tc + 1;
+    for (int f = 4; f >= 1; --f) {
+      for (int b = 0; b < npla; ++b) {
+        int fb = (f - 1) * npla + b;
+        size_t idx = (size_t)m->bfi[fb] + Nb * (time - 1);
+        dexpkv(m, f - 1, b, m->x[idx], de);
+        const zc *ei = EKV(m, m->ekvi, f - 1, b, time);
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) {
+            zc s = 0;
+            for (int k = 0; k < 4; ++k) s = s + de[i + 4 * k] * ei[k + 4 * j];
+            md[i + 4 * j] = s;
+          }
+        const int *S = &m->sites[fb * 4];
+        zc tr = 0;
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) tr = tr + md[i + 4 * j] * m->g[S[j] + (size_t)Ns * S[i]];
+        if (m->tr_weight) tr = tr + conj(tr);
+        grad[idx] = grad[idx] + tr * (double)m->ncopy;
+      }
+      orc_right_evolve(m, time, (double *)m->g, Ns, f, 1);
+    }
+    if (tc % m->ngroup == 0) orc_get_g_scratch(m, time, 0, NULL);
+  }
+}
+
 static float g_rands(orc_ran2 *s) {
   float r;
   orc_ran2_fill(s, 1, &r);

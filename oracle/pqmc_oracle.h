@@ -109,6 +109,11 @@ typedef struct {
   double distance; /* global_update_distance before the char_length scaling */
 } orc_global;
 void orc_cal_new_weight(orc_model *m, double *re_im);
+/* complex d ln_cw / dx for every field, grad[i + Nb (t-1)], from a backward
+ * pass with no updates (the chain ends where a backward sweep leaves it);
+ * d expkv / dx for one bond */
+void orc_gradient(orc_model *m, double *grad);
+void orc_get_dexpkv(const orc_model *m, int pf_id, int pla, double xv, double *out);
 int orc_update_global(orc_model *m, int kind, const orc_global *gp, orc_ran2 *rng);
 void orc_get_global_counters(const orc_model *m, long *accepted, long *rejected);
 /* the proposal of update_global alone (fields and -delt*dE), model left unchanged */

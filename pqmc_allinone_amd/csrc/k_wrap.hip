@@ -211,3 +211,73 @@ extern "C" hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, 
   hipLaunchKernelGGL(k_ekv_build, dim3(blocks), dim3(256), 0, s, *md, x, bfi, cross, ph, ekv, t_lo, t_hi);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Field gradient of ln_cw for one family at one slice (no counterpart in the
+// reference).  With G positioned so that E_f(t) is the outermost factor of
+// its right string (just before the backward sweep's wrap E^-1 G E),
+// d ln det / dx = Tr[G dE E^-1], restricted to the bond's four sites:
+// sum_ij (de e^-1)(i,j) G(S_j, S_i).  One thread per bond.  With
+// n = delt sqrt(|K|^2 + (|V| x)^2), s = sinh(n)/n and nd = delt^2 |V|^2 x:
+// d cosh(n)/dx = s nd, d s/dx = r nd with r = (n cosh n - sinh n)/n^3 (its
+// series 1/3 + n^2/30 below n = 1e-3), dh/dx = -delt V_coe V12.
+
+__global__ void k_field_grad(model_t md, const zd *__restrict__ G, const double *__restrict__ x,
+                             const int *__restrict__ bfi, const int *__restrict__ cross, const zd *__restrict__ ph,
+                             const int *__restrict__ sites, const ekv_t *__restrict__ ekv, zd *__restrict__ grad,
+                             int f, int t) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= md.npla) return;
+  const int fb = f * md.npla + b;
+  const size_t idx = (size_t)bfi[fb] + (size_t)md.Nb * (t - 1);
+  const double xv = x[idx];
+  const double ak = hypot(md.kcoe[f].x, md.kcoe[f].y);
+  const double av = hypot(md.vcoe[f].x, md.vcoe[f].y);
+  const double n = md.dtau * sqrt(ak * ak + (av * xv) * (av * xv));
+  const double nd = md.dtau * md.dtau * av * av * xv;
+  const double s = n > 0.0 ? sinh(n) / n : 1.0;
+  const double r = n < 1e-3 ? 1.0 / 3.0 + n * n / 30.0 : (n * cosh(n) - sinh(n)) / (n * n * n);
+  const zd *p = cross[fb] ? &ph[4 * fb] : nullptr;
+  const zd vx = zscale(md.vcoe[f], xv);
+  // dE = dc I + [[0, dm],[dm^H, 0]]
+  ekv_t de;
+  de.c = s * nd;
+  de.pad = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    zd h = zscale(zadd(zmul(md.kcoe[f], md.k12[f][q]), zmul(vx, md.v12[f][q])), -md.dtau);
+    zd dh = zscale(zmul(md.vcoe[f], md.v12[f][q]), -md.dtau);
+    if (p) {
+      h = zmul(h, p[q]);
+      dh = zmul(dh, p[q]);
+    }
+    de.m[q] = zadd(zscale(dh, s), zscale(h, r * nd));
+  }
+  const ekv_t e = ekv[((size_t)(t - 1) * 4 + f) * md.npla + b];
+  const int *S = sites + 4 * fb;
+  int st[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) st[i] = S[i];
+  zd tr = zmk(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      zd mij = zmk(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) mij = zadd(mij, zmul(ekv_entry(de, false, i, k), ekv_entry(e, true, k, j)));
+      tr = zadd(tr, zmul(mij, G[st[j] + (size_t)md.Ns * st[i]]));
+    }
+  if (md.tr_weight) tr = zmk(2.0 * tr.x, 0.0);
+  tr = zscale(tr, (double)md.ncopy);
+  grad[idx] = zadd(grad[idx], tr);
+}
+
+extern "C" hipError_t pqmc_launch_field_grad(const model_t *md, const zd *G, const double *x, const int *bfi,
+                                             const int *cross, const zd *ph, const int *sites, const ekv_t *ekv,
+                                             zd *grad, int f, int t, hipStream_t s) {
+  if (f < 0 || f > 3 || t < 1 || t > md->Lt) return hipErrorInvalidValue;
+  const int blocks = (md->npla + 63) / 64;
+  hipLaunchKernelGGL(k_field_grad, dim3(blocks), dim3(64), 0, s, *md, G, x, bfi, cross, ph, sites, ekv, grad, f, t);
+  return hipGetLastError();
+}

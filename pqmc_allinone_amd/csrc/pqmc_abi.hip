@@ -55,6 +55,9 @@ hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncol
 hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                      int tr_weight, int ncopy, zd *out, hipStream_t s);
 hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
+hipError_t pqmc_launch_field_grad(const model_t *md, const zd *G, const double *x, const int *bfi, const int *cross,
+                                  const zd *ph, const int *sites, const ekv_t *ekv, zd *grad, int f, int t,
+                                  hipStream_t s);
 hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
 hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
@@ -128,6 +131,7 @@ struct pqmc_handle {
   ps_rec *d_rec = nullptr;
   unsigned long long *d_psync = nullptr;
   double *d_lncw_acc = nullptr;
+  zd *d_grad = nullptr;  // field gradient, [Lt][Nb]
   zd *d_lncw = nullptr;
   unsigned long long *d_stats = nullptr, *d_err = nullptr;
   zd *d_meas = nullptr, *d_kphase = nullptr;
@@ -357,7 +361,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
-                  h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft};
+                  h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -970,6 +974,39 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass");
+  return PQMC_OK;
+}
+
+// Field gradient of ln_cw (no counterpart in the reference): a backward pass
+// without updates, the gradient kernel of each family before its two-way wrap
+// E^-1 G E, G from scratch every ngroup slices as in a backward sweep.
+extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
+  if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  const int Lt = h->Lt;
+  const size_t n = (size_t)h->Nb * Lt;
+  if (!h->d_grad) HIPCHK(dalloc(&h->d_grad, n));
+  HIPCHK(hipMemsetAsync(h->d_grad, 0, sizeof(zd) * n, h->st));
+  int rc = PQMC_OK;
+  for (int tc = 1; tc <= Lt; ++tc) {
+    const int time = Lt - tc + 1;
+    for (int f = 3; f >= 0; --f) {
+      HIPCHK(pqmc_launch_field_grad(&h->md, h->d_G, h->d_x, h->d_bfi, h->d_cross, h->d_ph, h->d_sites, h->d_ekv,
+                                    h->d_grad, f, time, h->st));
+      rc = wrap_g(h, 2, f, time);
+      if (rc) return rc;
+    }
+    if (tc % h->ngroup == 0) {
+      rc = g_scratch(h, time, 0);
+      if (rc) return rc;
+    }
+  }
+  if (Lt >= h->ngroup) {
+    rc = read_err(h, nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(grad, h->d_grad, sizeof(zd) * n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return PQMC_OK;
 }
 

@@ -179,6 +179,13 @@ module pqmc_hip
       real(c_double), intent(out) :: re_im(2)
     end function pqmc_global_weight
 
+    ! d ln_cw / dx for every field, complex, boson_field layout (Nb, Lt)
+    integer(c_int) function pqmc_gradient(h, grad) bind(C, name='pqmc_gradient')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(out) :: grad(*)
+    end function pqmc_gradient
+
     integer(c_int) function pqmc_get_accepts(h, acc, n) bind(C, name='pqmc_get_accepts')
       import :: c_int, c_ptr, c_int8_t, c_int64_t
       type(c_ptr), value :: h

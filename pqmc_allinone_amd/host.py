@@ -17,6 +17,7 @@ loop body (one sweep)        ``Chain.sweep(forward, draws, measure)``
 ran2_s / rands / ran_sysm    ``Ran2`` (src/mod_ranstate.f90:171-236)
 proj = .false. (_FT family)  ``Chain(model, finite_temperature=True)``
 cal_new_weight_T0 / _FT      ``Chain.global_weight(x)``
+(adjoint, no counterpart)    ``Chain.gradient()``, ``HybridMC``
 one chain per MPI rank       ``sweep_batch`` / ``sweep_many`` (chains together)
 ===========================  ============================================
 
@@ -280,6 +281,15 @@ class Chain:
         _lib.check(self._L.pqmc_global_weight(self.h, _lib.ptr(x), _lib.ptr(v)))
         return complex(v[0], v[1])
 
+    def gradient(self) -> np.ndarray:
+        """d ln_cw / dx for every field, complex, shape (Lt, Nb): a backward
+        pass with no updates (pqmc_gradient).  Call where a backward sweep
+        starts (after init_g(1) or a forward sweep); the chain ends where a
+        backward sweep that accepts nothing leaves it."""
+        g = np.empty((self.model.p.Lt, self.Nb), np.complex128)
+        _lib.check(self._L.pqmc_gradient(self.h, _lib.ptr(g)))
+        return g
+
     def q_string(self, block: int) -> np.ndarray:
         q = np.empty((self.Ns, self.model.p.ne), np.complex128, order="F")
         _lib.check(self._L.pqmc_get_qstring(self.h, int(block), _lib.ptr(q)))
@@ -439,3 +449,82 @@ class GlobalMoves:
             self.chain.set_fields(x)
             self.chain.init_g(1)
         return flags
+
+
+def bosonic_action(model: Model, x: np.ndarray) -> float:
+    """The phonon action whose differences cal_energy_difference
+    (src/mod_update.f90:203-228) takes: delt sum_t sum_i [M/(2 delt^2)
+    (x_t - x_(t-1))^2 + D/2 x_t^2 - biased x_t], periodic in t; x is (Lt, Nb)."""
+    dt = model.p.dtau
+    kin = 0.5 * model.M / dt ** 2 * ((x - np.roll(x, 1, axis=0)) ** 2).sum()
+    return float(dt * (kin + (0.5 * model.D * x ** 2 - model.p.biased * x).sum()))
+
+
+def bosonic_gradient(model: Model, x: np.ndarray) -> np.ndarray:
+    """d bosonic_action / dx, (Lt, Nb)."""
+    dt = model.p.dtau
+    lap = 2 * x - np.roll(x, 1, axis=0) - np.roll(x, -1, axis=0)
+    return dt * (model.M / dt ** 2 * lap + model.D * x - model.p.biased)
+
+
+def leapfrog(x: np.ndarray, p: np.ndarray, force, step: float, nsteps: int):
+    """nsteps leapfrog steps of dx/ds = p, dp/ds = force(x); reversible and
+    volume-preserving, so with an exact Metropolis test it leaves the target
+    distribution unchanged."""
+    x = x.copy()
+    p = p + 0.5 * step * force(x)
+    for k in range(nsteps):
+        x = x + step * p
+        p = p + (step if k + 1 < nsteps else 0.5 * step) * force(x)
+    return x, p
+
+
+class HybridMC:
+    """Hybrid-Monte-Carlo global phonon move (no counterpart in the
+    reference; it sits beside update_global, src/mod_update.f90:251-290).
+    The action is S(x) = bosonic_action(x) - Re ln_cw(x); the force
+    -dS/dx = -d bosonic/dx + Re(Chain.gradient()) drives a leapfrog
+    trajectory from Gaussian momenta, and the endpoint is accepted with the
+    exact full-determinant weight (Chain.global_weight) in the Metropolis
+    test min(1, exp(-dH)).  The chain is left after init_g(1), ready for a
+    backward sweep, as after the sweep loop's global moves
+    (src/Main_PQMC.f90:87-95).
+    """
+
+    def __init__(self, chain: Chain, step: float = 0.05, nsteps: int = 10, seed: int = 0):
+        self.chain, self.step, self.nsteps = chain, float(step), int(nsteps)
+        self.rng = np.random.default_rng(seed)
+        self.accepted = 0
+        self.rejected = 0
+        self.last_dH = 0.0
+
+    def force(self, x: np.ndarray) -> np.ndarray:
+        c = self.chain
+        c.set_fields(x)
+        c.init_g(1)
+        return c.gradient().real - bosonic_gradient(c.model, x)
+
+    def hamiltonian(self, x: np.ndarray, p: np.ndarray, lncw: complex) -> float:
+        return bosonic_action(self.chain.model, x) - lncw.real + 0.5 * float((p * p).sum())
+
+    def trajectory(self, x: np.ndarray, p: np.ndarray):
+        """(x, p) after one leapfrog trajectory; the chain's fields are left at x_end"""
+        return leapfrog(x, p, self.force, self.step, self.nsteps)
+
+    def update(self) -> bool:
+        c = self.chain
+        x0 = c.get_fields()
+        w0 = c.global_weight(x0)
+        p0 = self.rng.standard_normal(x0.shape)
+        x1, p1 = self.trajectory(x0, p0)
+        w1 = c.global_weight(x1)
+        dH = self.hamiltonian(x1, p1, w1) - self.hamiltonian(x0, p0, w0)
+        self.last_dH = dH
+        acc = bool(self.rng.random() < math.exp(min(0.0, -dH)))
+        c.set_fields(x1 if acc else x0)
+        c.init_g(1)
+        if acc:
+            self.accepted += 1
+        else:
+            self.rejected += 1
+        return acc

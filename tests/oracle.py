@@ -68,6 +68,8 @@ def lib():
         L.orc_zinverse.argtypes = [C.c_int, C.c_void_p]
         L.orc_zdet.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         L.orc_cal_new_weight.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_gradient.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_get_dexpkv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
         L.orc_update_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(GlobalParams), C.POINTER(Ran2State)]
         L.orc_update_global.restype = C.c_int
         L.orc_get_global_counters.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -233,6 +235,17 @@ class Oracle:
         out = np.zeros(2)
         lib().orc_cal_new_weight(self.m, P(out))
         return complex(out[0], out[1])
+
+    def gradient(self):
+        """complex d ln_cw / dx, shape (Lt, Nb)"""
+        g = np.zeros((self.Lt, self.Nb), np.complex128)
+        lib().orc_gradient(self.m, P(g))
+        return g
+
+    def dexpkv(self, pf_id, pla, x):
+        e = np.zeros((4, 4), np.complex128, order="F")
+        lib().orc_get_dexpkv(self.m, int(pf_id), int(pla), float(x), P(e))
+        return e
 
     def update_global(self, kind, rng, distance=1.0):
         gp = GlobalParams(float(distance))
