@@ -150,6 +150,50 @@ def ft_leg(m, x, seed, warmup, steps):
     return out
 
 
+def grad_leg(m, x, sl, passes, nchains):
+    """The field gradient d ln_cw / dx (pqmc_gradient) on the same lattice and
+    fields: one chain alone, then nchains chains at once (one host thread
+    each, their own streams), each pass from a fresh init_g(1), which is not
+    timed.  A hybrid-Monte-Carlo force evaluation is one init_g plus one
+    pass."""
+    from concurrent.futures import ThreadPoolExecutor
+    from pqmc_allinone_amd.host import Chain
+    nf = m.Nb * m.p.Lt
+
+    def chain(k):
+        ch = Chain(m)
+        ch.set_fields(x if k == 0 else x + 1e-3 * k)
+        ch.set_trial(sl)
+        return ch
+
+    ch = chain(0)
+    t_init, t_pass = [], []
+    for _ in range(passes):
+        t0 = time.perf_counter()
+        ch.init_g(1)
+        ch.synchronize()
+        t1 = time.perf_counter()
+        ch.gradient()
+        t_pass.append(time.perf_counter() - t1)
+        t_init.append(t1 - t0)
+    ch.close()
+    out = {"fields": nf, "ms_per_pass": 1e3 * min(t_pass), "init_ms": 1e3 * min(t_init),
+           "fields_per_s": nf / min(t_pass), "dtype": "c128 (fp64)"}
+    if nchains > 1:
+        chs = [chain(k) for k in range(nchains)]
+        with ThreadPoolExecutor(nchains) as pool:
+            list(pool.map(lambda c: c.init_g(1), chs))
+            for c in chs:
+                c.synchronize()
+            t0 = time.perf_counter()
+            list(pool.map(lambda c: c.gradient(), chs))
+            dt = time.perf_counter() - t0
+        for c in chs:
+            c.close()
+        out["chains"] = {"n": nchains, "ms": 1e3 * dt, "fields_per_s": nchains * nf / dt}
+    return out
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -180,6 +224,9 @@ def main():
                     help="bonds per column-owning workgroup in the batched leg's family passes (1, 2 or 4)")
     ap.add_argument("--ft-steps", type=int, default=2,
                     help="timed sweeps of the finite-temperature leg (0: no such leg)")
+    ap.add_argument("--grad-passes", type=int, default=2,
+                    help="field-gradient passes on one chain (0: skip the gradient leg)")
+    ap.add_argument("--grad-chains", type=int, default=16, help="chains of the concurrent gradient pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
@@ -263,6 +310,9 @@ def main():
     ft = None
     if args.ft_steps > 0:
         ft = ft_leg(m, x, seed, args.warmup, args.ft_steps)
+    grad = None
+    if args.grad_passes > 0:
+        grad = grad_leg(m, x, sl, args.grad_passes, args.grad_chains)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
@@ -288,6 +338,7 @@ def main():
         "kernels": kern,
         "batched": batched,
         "finite_temperature": ft,
+        "gradient": grad,
         "cpu_baseline": cpu,
         "speedup_vs_cpu_core": (sps / cpu["value"]) if cpu else None,
     }
