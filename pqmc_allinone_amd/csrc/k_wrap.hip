@@ -10,20 +10,13 @@
 //    (init_expKV, src/mod_phonon_field.f90:423-449).
 #include "pqmc_common.h"
 
+PQ_DEV zd bond_field_grad(const model_t &md, int f, double xv, const zd *p, const ekv_t &e, const zd g[16]);
+
 #define FW_TB 8   // bonds per tile side
 #define FW_TS 32  // sites per tile side
 #define FW_KC 16  // rank chunk
 
-struct fw_args {
-  model_t md;
-  zd *G;
-  const zd *U, *VT;     // flush factors (row-major, ld Ns), family-order rows of the flush family
-  const int *rank;      // device rank r (null: no flush)
-  const int *pos_ffl;   // [Ns] site -> family-order row of the flush family
-  const int *tsites;    // [npla*4] sites of the tiling family
-  int wrap;             // 0 none, 1 left: E G E^-1, 2 right: E^-1 G E
-  const ekv_t *ekv_w;   // [npla] exponentials of the wrap family at the wrap slice
-};
+
 
 __global__ __launch_bounds__(256) void k_flush_wrap(fw_args A) {
   const int npla = A.md.npla, Ns = A.md.Ns;
@@ -88,6 +81,21 @@ __global__ __launch_bounds__(256) void k_flush_wrap(fw_args A) {
     if (ii < nr && jj < nc) sT[ii][jj] = zadd(sT[ii][jj], A.G[rs[ii] + (size_t)Ns * cs[jj]]);
   }
   __syncthreads();
+  if (A.grad) {  // uniform: a kernel argument
+    // a diagonal tile holds each of its bonds' G(S, S) before the wrap
+    if (blockIdx.x == blockIdx.y && tid < nbr) {
+      const int b = bi0 + tid, fb = A.f_wrap * npla + b;
+      const size_t idx = (size_t)A.bfi[fb] + (size_t)A.md.Nb * (A.t_wrap - 1);
+      zd g[16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[i + 4 * j] = sT[4 * tid + i][4 * tid + j];
+      const zd tr = bond_field_grad(A.md, A.f_wrap, A.x[idx], A.cross[fb] ? &A.ph[4 * fb] : nullptr, A.ekv_w[b], g);
+      A.grad[idx] = zadd(A.grad[idx], tr);
+    }
+    __syncthreads();
+  }
   if (A.wrap) {
     const bool left = (A.wrap == 1);
     {  // rows: thread (bond bi, column jj): v = B(:, jj) of block row bi -> E_i v (left) / E_i^-1 v
@@ -217,27 +225,21 @@ extern "C" hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, 
 // reference).  With G positioned so that E_f(t) is the outermost factor of
 // its right string (just before the backward sweep's wrap E^-1 G E),
 // d ln det / dx = Tr[G dE E^-1], restricted to the bond's four sites:
-// sum_ij (de e^-1)(i,j) G(S_j, S_i).  One thread per bond.  With
+// sum_ij (de e^-1)(i,j) G(S_j, S_i), one thread per bond of a diagonal tile
+// of k_flush_wrap, from the tile's G before the wrap.  With
 // n = delt sqrt(|K|^2 + (|V| x)^2), s = sinh(n)/n and nd = delt^2 |V|^2 x:
 // d cosh(n)/dx = s nd, d s/dx = r nd with r = (n cosh n - sinh n)/n^3 (its
 // series 1/3 + n^2/30 below n = 1e-3), dh/dx = -delt V_coe V12.
 
-__global__ void k_field_grad(model_t md, const zd *__restrict__ G, const double *__restrict__ x,
-                             const int *__restrict__ bfi, const int *__restrict__ cross, const zd *__restrict__ ph,
-                             const int *__restrict__ sites, const ekv_t *__restrict__ ekv, zd *__restrict__ grad,
-                             int f, int t) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= md.npla) return;
-  const int fb = f * md.npla + b;
-  const size_t idx = (size_t)bfi[fb] + (size_t)md.Nb * (t - 1);
-  const double xv = x[idx];
+// d ln det / dx of one bond from its G(S, S) block g (column-major 4x4),
+// tr_weight doubling and ncopy applied.
+PQ_DEV zd bond_field_grad(const model_t &md, int f, double xv, const zd *p, const ekv_t &e, const zd g[16]) {
   const double ak = hypot(md.kcoe[f].x, md.kcoe[f].y);
   const double av = hypot(md.vcoe[f].x, md.vcoe[f].y);
   const double n = md.dtau * sqrt(ak * ak + (av * xv) * (av * xv));
   const double nd = md.dtau * md.dtau * av * av * xv;
   const double s = n > 0.0 ? sinh(n) / n : 1.0;
   const double r = n < 1e-3 ? 1.0 / 3.0 + n * n / 30.0 : (n * cosh(n) - sinh(n)) / (n * n * n);
-  const zd *p = cross[fb] ? &ph[4 * fb] : nullptr;
   const zd vx = zscale(md.vcoe[f], xv);
   // dE = dc I + [[0, dm],[dm^H, 0]]
   ekv_t de;
@@ -253,11 +255,6 @@ __global__ void k_field_grad(model_t md, const zd *__restrict__ G, const double 
     }
     de.m[q] = zadd(zscale(dh, s), zscale(h, r * nd));
   }
-  const ekv_t e = ekv[((size_t)(t - 1) * 4 + f) * md.npla + b];
-  const int *S = sites + 4 * fb;
-  int st[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) st[i] = S[i];
   zd tr = zmk(0.0, 0.0);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -266,18 +263,8 @@ __global__ void k_field_grad(model_t md, const zd *__restrict__ G, const double 
       zd mij = zmk(0.0, 0.0);
 #pragma unroll
       for (int k = 0; k < 4; ++k) mij = zadd(mij, zmul(ekv_entry(de, false, i, k), ekv_entry(e, true, k, j)));
-      tr = zadd(tr, zmul(mij, G[st[j] + (size_t)md.Ns * st[i]]));
+      tr = zadd(tr, zmul(mij, g[j + 4 * i]));
     }
   if (md.tr_weight) tr = zmk(2.0 * tr.x, 0.0);
-  tr = zscale(tr, (double)md.ncopy);
-  grad[idx] = zadd(grad[idx], tr);
-}
-
-extern "C" hipError_t pqmc_launch_field_grad(const model_t *md, const zd *G, const double *x, const int *bfi,
-                                             const int *cross, const zd *ph, const int *sites, const ekv_t *ekv,
-                                             zd *grad, int f, int t, hipStream_t s) {
-  if (f < 0 || f > 3 || t < 1 || t > md->Lt) return hipErrorInvalidValue;
-  const int blocks = (md->npla + 63) / 64;
-  hipLaunchKernelGGL(k_field_grad, dim3(blocks), dim3(64), 0, s, *md, G, x, bfi, cross, ph, sites, ekv, grad, f, t);
-  return hipGetLastError();
+  return zscale(tr, (double)md.ncopy);
 }

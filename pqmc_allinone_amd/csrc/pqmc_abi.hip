@@ -20,16 +20,6 @@
 #include "k_pass.h"
 
 // ---- kernel launchers (k_*.hip) ----
-struct fw_args {
-  model_t md;
-  zd *G;
-  const zd *U, *VT;
-  const int *rank;
-  const int *pos_ffl;
-  const int *tsites;
-  int wrap;
-  const ekv_t *ekv_w;
-};
 struct prop_args {
   model_t md;
   zd *X;
@@ -55,9 +45,6 @@ hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncol
 hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                      int tr_weight, int ncopy, zd *out, hipStream_t s);
 hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
-hipError_t pqmc_launch_field_grad(const model_t *md, const zd *G, const double *x, const int *bfi, const int *cross,
-                                  const zd *ph, const int *sites, const ekv_t *ekv, zd *grad, int f, int t,
-                                  hipStream_t s);
 hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
 hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
@@ -520,8 +507,8 @@ extern "C" int pqmc_set_trial(pqmc_handle *h, const double *slater) {
 }
 
 // standalone two-way wrap of G (wrap 1: E G E^-1, 2: E^-1 G E)
-static int wrap_g(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
-  fw_args a;
+static int wrap_g(pqmc_handle *h, int wrap, int f_wrap, int t_wrap, zd *grad = nullptr) {
+  fw_args a{};
   a.md = h->md;
   a.G = h->d_G;
   a.U = nullptr;
@@ -531,6 +518,15 @@ static int wrap_g(pqmc_handle *h, int wrap, int f_wrap, int t_wrap) {
   a.tsites = h->d_sites + (size_t)f_wrap * h->npla * 4;
   a.wrap = wrap;
   a.ekv_w = h->d_ekv + ((size_t)(t_wrap - 1) * 4 + f_wrap) * h->npla;
+  if (grad) {
+    a.grad = grad;
+    a.x = h->d_x;
+    a.bfi = h->d_bfi;
+    a.cross = h->d_cross;
+    a.ph = h->d_ph;
+    a.f_wrap = f_wrap;
+    a.t_wrap = t_wrap;
+  }
   span_guard sg(h, 1);
   HIPCHK(pqmc_launch_flush_wrap(&a, h->st));
   return PQMC_OK;
@@ -809,7 +805,7 @@ extern "C" int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left,
   if (two_way) {
     // run the two-way wrap on a scratch copy in d_Gn
     HIPCHK(hipMemcpyAsync(h->d_Gn, mat, sizeof(zd) * Ns * Ns, hipMemcpyHostToDevice, h->st));
-    fw_args a;
+    fw_args a{};
     a.md = h->md;
     a.G = h->d_Gn;
     a.U = a.VT = nullptr;
@@ -978,8 +974,9 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
 }
 
 // Field gradient of ln_cw (no counterpart in the reference): a backward pass
-// without updates, the gradient kernel of each family before its two-way wrap
-// E^-1 G E, G from scratch every ngroup slices as in a backward sweep.
+// without updates; each family's two-way wrap E^-1 G E first adds the
+// family's gradient from the G(S, S) blocks it holds (k_flush_wrap's diagonal
+// tiles), and G is recomputed every ngroup slices as in a backward sweep.
 extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
   if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
@@ -991,9 +988,7 @@ extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
   for (int tc = 1; tc <= Lt; ++tc) {
     const int time = Lt - tc + 1;
     for (int f = 3; f >= 0; --f) {
-      HIPCHK(pqmc_launch_field_grad(&h->md, h->d_G, h->d_x, h->d_bfi, h->d_cross, h->d_ph, h->d_sites, h->d_ekv,
-                                    h->d_grad, f, time, h->st));
-      rc = wrap_g(h, 2, f, time);
+      rc = wrap_g(h, 2, f, time, h->d_grad);
       if (rc) return rc;
     }
     if (tc % h->ngroup == 0) {

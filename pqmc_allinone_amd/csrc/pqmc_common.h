@@ -46,6 +46,25 @@ struct model_t {
   zd bc[2];                 // lattice boundary phases
 };
 
+// Arguments of k_flush_wrap (the delayed flush fused with a two-way wrap).
+struct fw_args {
+  model_t md;
+  zd *G;
+  const zd *U, *VT;     // flush factors (row-major, ld Ns), family-order rows of the flush family
+  const int *rank;      // device rank r (null: no flush)
+  const int *pos_ffl;   // [Ns] site -> family-order row of the flush family
+  const int *tsites;    // [npla*4] sites of the tiling family
+  int wrap;             // 0 none, 1 left: E G E^-1, 2 right: E^-1 G E
+  const ekv_t *ekv_w;   // [npla] exponentials of the wrap family at the wrap slice
+  // accumulate the wrap family's field gradient from G before the wrap
+  // (null: none), bond_field_grad of k_wrap.hip
+  zd *grad;
+  const double *x;
+  const int *bfi, *cross;
+  const zd *ph;
+  int f_wrap, t_wrap;
+};
+
 // get_expKV_epsocz (src/mod_phonon_field.f90:534-569) in compact form.
 // ph points at the bond's BC_phases(1:2,3:4) (4 values) or is null.
 PQ_DEV void expkv_compact(const model_t &md, int f, double x, const zd *ph, ekv_t &e) {
