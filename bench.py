@@ -194,6 +194,28 @@ def grad_leg(m, x, sl, passes, nchains):
     return out
 
 
+def hmc_leg(m, x, sl, ntraj):
+    """Hybrid-Monte-Carlo moves on one chain of the same workload: leapfrog
+    trajectories of 10 steps of 0.05 with the Fourier-accelerated mass, each
+    force one init_g(1) plus one gradient pass, the endpoint weighed by the
+    exact full-determinant weight."""
+    from pqmc_allinone_amd.host import Chain, HybridMC
+    ch = Chain(m)
+    ch.set_fields(x)
+    ch.set_trial(sl)
+    ch.init_g(1)
+    hmc = HybridMC(ch, step=0.05, nsteps=10, seed=1)
+    dts, dhs = [], []
+    for _ in range(ntraj):
+        t0 = time.perf_counter()
+        hmc.update()
+        dts.append(time.perf_counter() - t0)
+        dhs.append(hmc.last_dH)
+    ch.close()
+    return {"trajectories": ntraj, "s_per_trajectory": float(np.mean(dts)), "dH": dhs,
+            "accepted": hmc.accepted, "leapfrog": "10 x 0.05, Fourier-accelerated mass"}
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -227,6 +249,7 @@ def main():
     ap.add_argument("--grad-passes", type=int, default=2,
                     help="field-gradient passes on one chain (0: skip the gradient leg)")
     ap.add_argument("--grad-chains", type=int, default=16, help="chains of the concurrent gradient pass")
+    ap.add_argument("--hmc-trajectories", type=int, default=1, help="HMC trajectories (0: skip the HMC leg)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
@@ -313,6 +336,9 @@ def main():
     grad = None
     if args.grad_passes > 0:
         grad = grad_leg(m, x, sl, args.grad_passes, args.grad_chains)
+    hmc = None
+    if args.hmc_trajectories > 0:
+        hmc = hmc_leg(m, x, sl, args.hmc_trajectories)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
@@ -339,6 +365,7 @@ def main():
         "batched": batched,
         "finite_temperature": ft,
         "gradient": grad,
+        "hmc": hmc,
         "cpu_baseline": cpu,
         "speedup_vs_cpu_core": (sps / cpu["value"]) if cpu else None,
     }

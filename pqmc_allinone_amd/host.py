@@ -467,16 +467,48 @@ def bosonic_gradient(model: Model, x: np.ndarray) -> np.ndarray:
     return dt * (model.M / dt ** 2 * lap + model.D * x - model.p.biased)
 
 
-def leapfrog(x: np.ndarray, p: np.ndarray, force, step: float, nsteps: int):
-    """nsteps leapfrog steps of dx/ds = p, dp/ds = force(x); reversible and
+def leapfrog(x: np.ndarray, p: np.ndarray, force, step: float, nsteps: int, minv=None):
+    """nsteps leapfrog steps of dx/ds = minv(p), dp/ds = force(x) (minv:
+    the inverse mass matrix applied to p, identity if None); reversible and
     volume-preserving, so with an exact Metropolis test it leaves the target
     distribution unchanged."""
+    minv = (lambda q: q) if minv is None else minv
     x = x.copy()
     p = p + 0.5 * step * force(x)
     for k in range(nsteps):
-        x = x + step * p
+        x = x + step * minv(p)
         p = p + (step if k + 1 < nsteps else 0.5 * step) * force(x)
     return x, p
+
+
+class BosonicMass:
+    """Fourier-accelerated mass matrix: the Hessian of bosonic_action,
+    K = delt (M/delt^2 (2 - shift - shift^-1) + D) per field, circulant in
+    time, so that every free phonon mode moves at unit frequency and the
+    leapfrog error no longer grows with M/delt^2.  Its time-Fourier
+    eigenvalues are delt (M/delt^2 (2 - 2 cos(2 pi k/Lt)) + D); D > 0 keeps
+    them positive."""
+
+    def __init__(self, model: Model):
+        dt, Lt = model.p.dtau, model.p.Lt
+        if not model.D > 0:
+            raise ValueError("the bosonic mass matrix needs D > 0")
+        k = np.arange(Lt // 2 + 1)
+        self.khat = dt * (model.M / dt ** 2 * (2 - 2 * np.cos(2 * np.pi * k / Lt)) + model.D)
+        self.Lt = Lt
+
+    def _apply(self, v, w):
+        return np.fft.irfft(w[:, None] * np.fft.rfft(v, axis=0), n=self.Lt, axis=0)
+
+    def inverse(self, p):
+        return self._apply(p, 1.0 / self.khat)
+
+    def sample(self, rng, shape):
+        """p ~ N(0, K)"""
+        return self._apply(rng.standard_normal(shape), np.sqrt(self.khat))
+
+    def kinetic(self, p):
+        return 0.5 * float((p * self.inverse(p)).sum())
 
 
 class HybridMC:
@@ -486,14 +518,21 @@ class HybridMC:
     -dS/dx = -d bosonic/dx + Re(Chain.gradient()) drives a leapfrog
     trajectory from Gaussian momenta, and the endpoint is accepted with the
     exact full-determinant weight (Chain.global_weight) in the Metropolis
-    test min(1, exp(-dH)).  The chain is left after init_g(1), ready for a
-    backward sweep, as after the sweep loop's global moves
-    (src/Main_PQMC.f90:87-95).
+    test min(1, exp(-dH)).  mass = "bosonic" (the default when D > 0) uses
+    the Fourier-accelerated BosonicMass, "unit" the identity.  The chain is
+    left after init_g(1), ready for a backward sweep, as after the sweep
+    loop's global moves (src/Main_PQMC.f90:87-95).
     """
 
-    def __init__(self, chain: Chain, step: float = 0.05, nsteps: int = 10, seed: int = 0):
+    def __init__(self, chain: Chain, step: float = 0.05, nsteps: int = 10, seed: int = 0,
+                 mass: str | None = None):
         self.chain, self.step, self.nsteps = chain, float(step), int(nsteps)
         self.rng = np.random.default_rng(seed)
+        if mass is None:
+            mass = "bosonic" if chain.model.D > 0 else "unit"
+        if mass not in ("bosonic", "unit"):
+            raise ValueError("mass must be 'bosonic' or 'unit'")
+        self.mass = BosonicMass(chain.model) if mass == "bosonic" else None
         self.accepted = 0
         self.rejected = 0
         self.last_dH = 0.0
@@ -504,18 +543,24 @@ class HybridMC:
         c.init_g(1)
         return c.gradient().real - bosonic_gradient(c.model, x)
 
+    def kinetic(self, p: np.ndarray) -> float:
+        return self.mass.kinetic(p) if self.mass else 0.5 * float((p * p).sum())
+
+    def momenta(self, shape) -> np.ndarray:
+        return self.mass.sample(self.rng, shape) if self.mass else self.rng.standard_normal(shape)
+
     def hamiltonian(self, x: np.ndarray, p: np.ndarray, lncw: complex) -> float:
-        return bosonic_action(self.chain.model, x) - lncw.real + 0.5 * float((p * p).sum())
+        return bosonic_action(self.chain.model, x) - lncw.real + self.kinetic(p)
 
     def trajectory(self, x: np.ndarray, p: np.ndarray):
         """(x, p) after one leapfrog trajectory; the chain's fields are left at x_end"""
-        return leapfrog(x, p, self.force, self.step, self.nsteps)
+        return leapfrog(x, p, self.force, self.step, self.nsteps, self.mass.inverse if self.mass else None)
 
     def update(self) -> bool:
         c = self.chain
         x0 = c.get_fields()
         w0 = c.global_weight(x0)
-        p0 = self.rng.standard_normal(x0.shape)
+        p0 = self.momenta(x0.shape)
         x1, p1 = self.trajectory(x0, p0)
         w1 = c.global_weight(x1)
         dH = self.hamiltonian(x1, p1, w1) - self.hamiltonian(x0, p0, w0)
