@@ -902,13 +902,24 @@ __global__ void k_maxdiff(const zd *a, const zd *b, long n, unsigned long long *
     const double v = hypot(d.x, d.y);
     if (!(v <= m)) m = v;  // NaN propagates
   }
+  if (m != m) m = INFINITY;  // fmax in the wave reduction would drop a NaN
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  // one atomic per workgroup: thousands on one address serialize
+  __shared__ double sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      if (sm[w] > m) m = sm[w];
+    }
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 extern "C" hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s) {
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 1024) blocks = 1024;
+  int blocks = (int)((n + 1023) / 1024);
+  if (blocks > 256) blocks = 256;
+  if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_maxdiff, dim3(blocks), dim3(256), 0, s, a, b, n, out);
   return hipGetLastError();
 }
