@@ -300,3 +300,28 @@ def test_bosonic_mass_is_the_action_hessian():
     d_unit = abs(bosonic_action(m, x2) + 0.5 * (p2 ** 2).sum() - bosonic_action(m, x0) - 0.5 * (p_unit ** 2).sum())
     print(f"|dH| Fourier-accelerated {d_acc:.3e}, unit mass {d_unit:.3e}")
     assert d_acc < 0.05 and d_unit > 10 * d_acc
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_of_concurrent_chains_match_serial():
+    """Four chains of different fields computing their gradients at once, one
+    host thread and one stream each, give bit for bit the gradients each
+    computes alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    p = ModelParams(L1=6, L2=6, Lt=40, ngroup=10, ne=36)
+    m = build_model(p)
+    chains, serial = [], []
+    for k in range(4):
+        x, sl = synthetic_inputs(m, 20 + k)
+        ch = gpu_chain(m, x, sl, False)
+        ch.init_g(1)
+        serial.append(ch.gradient())
+        ch.init_g(1)
+        chains.append(ch)
+    print("concurrent gradients of 4 chains", flush=True)
+    with ThreadPoolExecutor(4) as pool:
+        conc = list(pool.map(lambda c: c.gradient(), chains))
+    for a, b in zip(serial, conc):
+        assert np.array_equal(a, b)
+    for ch in chains:
+        ch.close()
