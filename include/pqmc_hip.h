@@ -174,7 +174,9 @@ int pqmc_global_weight(pqmc_handle *h, const double *x, double *re_im);
  * beside update_global (src/mod_update.f90:251-290).  Starts where a
  * backward pqmc_sweep starts (after pqmc_init_g(1) or a forward sweep) and
  * leaves the chain where a backward sweep that accepts nothing leaves it.
- * Projector and finite-temperature chains alike. */
+ * Projector and finite-temperature chains alike.  Refused (PQMC_EINVAL)
+ * when G is known to sit at the bottom: after a full backward sweep or a
+ * gradient pass, with no init_g or forward sweep since. */
 int pqmc_gradient(pqmc_handle *h, double *grad);
 
 /* The ran2 generator of src/mod_ranstate.f90:171-191 seeded as

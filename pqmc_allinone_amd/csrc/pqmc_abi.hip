@@ -95,6 +95,7 @@ struct pqmc_handle {
   model_t md{};
   hipStream_t st = nullptr;
   bool model_set = false, trial_set = false, fields_set = false, g_ready = false;
+  int g_at = 0;  // where G sits: 0 not tracked, 1 top (after init_g or a full forward sweep), 2 bottom
   // device model tables
   int *d_sites = nullptr, *d_bfi = nullptr, *d_cross = nullptr, *d_pos = nullptr;
   zd *d_ph = nullptr;
@@ -688,6 +689,7 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->st));
     h->g_ready = true;
+    h->g_at = 1;
     return PQMC_OK;
   }
   const size_t qb = sizeof(zd) * Ns * ne;
@@ -723,6 +725,7 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   h->g_ready = true;
+  h->g_at = 1;
   return PQMC_OK;
 }
 
@@ -790,6 +793,7 @@ extern "C" int pqmc_evolve_g(pqmc_handle *h, int time, int pf_id, int forward) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
   if (pf_id < 1 || pf_id > 4 || time < 1 || time > h->Lt) return fail(PQMC_EINVAL, "pf_id or time out of range");
+  h->g_at = 0;
   return wrap_g(h, forward ? 1 : 2, pf_id - 1, time);
 }
 
@@ -970,6 +974,7 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass");
+  h->g_at = ntc == Lt ? (forward ? 1 : 2) : 0;
   return PQMC_OK;
 }
 
@@ -980,6 +985,9 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
 extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
   if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  if (h->g_at == 2)
+    return fail(PQMC_EINVAL, "G is at the bottom (after a backward sweep or a gradient pass): the gradient pass "
+                             "starts where a backward sweep starts, after init_g or a forward sweep");
   const int Lt = h->Lt;
   const size_t n = (size_t)h->Nb * Lt;
   if (!h->d_grad) HIPCHK(dalloc(&h->d_grad, n));
@@ -1002,6 +1010,7 @@ extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
   }
   HIPCHK(hipMemcpyAsync(grad, h->d_grad, sizeof(zd) * n, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  h->g_at = 2;
   return PQMC_OK;
 }
 
@@ -1144,6 +1153,7 @@ extern "C" int pqmc_set_g(pqmc_handle *h, const double *g) {
   if (!h || !g) return fail(PQMC_EINVAL, "null argument");
   HIPCHK(hipMemcpyAsync(h->d_G, g, sizeof(zd) * h->Ns * h->Ns, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  h->g_at = 0;
   return PQMC_OK;
 }
 

@@ -176,6 +176,8 @@ def test_gpu_gradient_matches_oracle(case):
     print(f"  rel grad err {e:.3e}  max|grad| {np.abs(g_ref).max():.3e}", flush=True)
     assert e < 1e-10
     assert rel(ch.g_h, orc.g) < 1e-10
+    with pytest.raises(Exception, match="bottom"):
+        ch.gradient()  # G now sits where a backward sweep ends
     u = O2.Ran2Stream(31).draws(2 * m.Nb * m.p.Lt)
     acc_o, _ = orc.sweep(True, u)
     ch.sweep(True, u)
