@@ -254,12 +254,13 @@ def main():
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
     args = ap.parse_args()
-    if args.batch > 1:
-        # one hardware queue per chain of the batched leg (read when the HIP
-        # runtime starts, so before the first device call)
+    # one hardware queue per chain of the batched and gradient legs (read when
+    # the HIP runtime starts, so before the first device call)
+    nq = max(args.batch if args.batch > 1 else 0, args.grad_chains if args.grad_passes > 0 else 0)
+    if nq > 1:
         q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-        if q < args.batch + 2:
-            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.batch + 3))
+        if q < nq + 2:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, nq + 3))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
