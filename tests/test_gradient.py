@@ -186,15 +186,16 @@ def test_gpu_gradient_matches_oracle(case):
 
 
 @pytest.mark.gpu
-def test_gpu_gradient_matches_finite_differences_at_14x14_lt1000():
+@pytest.mark.parametrize("ft", [False, True], ids=["projector", "finite_temperature"])
+def test_gpu_gradient_matches_finite_differences_at_14x14_lt1000(ft):
     """The 392,000-field chain (14x14, Lt=1000, ngroup 10): central differences
     of pqmc_global_weight at eps = 1e-4 on 8 seeded fields agree with the
     gradient to 1e-6 relative (plus the differences' rounding floor)."""
     p = ModelParams(L1=14, L2=14, Lt=1000, ngroup=10, ne=196)
     m = build_model(p)
     x, sl = synthetic_inputs(m, 7)
-    ch = gpu_chain(m, x, sl, False)
-    print("gradient 14x14 Lt=1000: init", flush=True)
+    ch = gpu_chain(m, x, sl, ft)
+    print(f"gradient 14x14 Lt=1000 ft={ft}: init", flush=True)
     ch.init_g(1)
     g = ch.gradient().real
     assert g.shape == (1000, m.Nb) and np.isfinite(g).all()
