@@ -221,7 +221,8 @@ def test_gpu_hybrid_mc_trajectory():
     """With the device gradient: a reversed trajectory returns to its start,
     the energy error falls about 4x per halved step, and HMC moves with the
     exact-determinant test are accepted and leave a chain whose G matches a
-    fresh init of the oracle at the new fields."""
+    fresh init of the oracle at the new fields, and a local backward sweep
+    from there keeps the oracle's accept sequence."""
     from pqmc_allinone_amd.host import HybridMC
     m, x, sl, orc = make(4, 20, 5, 16, False, 1, (0.0, 0.0), False)
     ch = gpu_chain(m, x, sl, False)
@@ -253,6 +254,13 @@ def test_gpu_hybrid_mc_trajectory():
     xf = ch.get_fields()
     orc.set_fields(xf)
     orc.init(1)
+    assert rel(ch.g_h, orc.g) < 1e-10
+    # mixed with the local sweeps: the chain is ready for a backward sweep
+    import oracle as O2
+    u = O2.Ran2Stream(77).draws(2 * m.Nb * m.p.Lt)
+    acc_o, _ = orc.sweep(False, u)
+    ch.sweep(False, u)
+    assert (ch.accepts() == acc_o).all() and acc_o.any()
     assert rel(ch.g_h, orc.g) < 1e-10
     ch.close()
 
