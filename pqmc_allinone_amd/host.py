@@ -451,6 +451,44 @@ class GlobalMoves:
         return flags
 
 
+def autotune_pass(model: Model, x: np.ndarray, slater: np.ndarray | None = None,
+                  block_bonds=(2, 4, 6, 8), helper_bonds=(1, 2, 4), slices: int = 20,
+                  finite_temperature: bool = False, seed: int = 1):
+    """Pick the family pass's shape for one lattice by timing it (no
+    reference counterpart).  For every (block_bonds, helper_bonds) pair that
+    the device accepts, a chain on these fields runs the first `slices`
+    slices of a backward sweep from init_g(1) twice, the first as a warm-up;
+    the second is timed.  Every shape gives the same accept sequence, so the
+    choice only moves time.  Returns ((block_bonds, helper_bonds), {pair:
+    seconds})."""
+    import time
+    u = Ran2(seed).draws(2 * model.Nb * model.p.Lt)
+    times = {}
+    for hb in helper_bonds:
+        for bb in block_bonds:
+            try:
+                ch = Chain(model, block_bonds=bb, helper_bonds=hb, finite_temperature=finite_temperature)
+            except _lib.PQMCError:
+                continue
+            try:
+                ch.set_fields(x)
+                if not finite_temperature:
+                    ch.set_trial(slater)
+                for rep in range(2):
+                    ch.init_g(1)
+                    ch.synchronize()
+                    t0 = time.perf_counter()
+                    ch.sweep(False, u, ntc=slices)
+                    ch.synchronize()
+                    dt = time.perf_counter() - t0
+                times[(bb, hb)] = dt
+            finally:
+                ch.close()
+    if not times:
+        raise ValueError("no family-pass shape fits the device for this lattice")
+    return min(times, key=times.get), times
+
+
 def bosonic_action(model: Model, x: np.ndarray) -> float:
     """The phonon action whose differences cal_energy_difference
     (src/mod_update.f90:203-228) takes: delt sum_t sum_i [M/(2 delt^2)
