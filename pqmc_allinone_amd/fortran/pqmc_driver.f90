@@ -5,7 +5,7 @@
 ! get_g_scratch every ngroup slices, begin_measure around ntime/2 after the
 ! warm-up, src/Main_PQMC.f90:143-148).
 !
-! usage: pqmc_driver <input file> <output file> [<output folder>/]
+! usage: pqmc_driver <input file> <output file> [<output folder>/ [<gradient file>]]
 ! The output folder (with temp/ and data/ in it) receives the measurement
 ! bins and files when nbins > 0, and checkpoint.bin when a checkpoint is
 ! saved or resumed from.
@@ -24,6 +24,8 @@
 ! fields(Nb, Lt); complex128 summed measurements (total_width); int32 number
 ! of measurements; int64 positive and negative accepts; int64 global
 ! accepts and rejects.
+! Gradient file (optional): complex128 d ln_cw / dx (Nb, Lt) of the final
+! fields, from init_g(1) and one gradient pass after the output is written.
 program pqmc_driver
   use iso_c_binding
   use pqmc_hip
@@ -33,6 +35,8 @@ program pqmc_driver
   integer :: u_in, u_out
   integer(c_int32_t) :: hdr(8), ncopy_tr(2), glob_loop, mc(8)
   character(len=1024) :: outdir
+  character(len=1024) :: fgrad
+  complex(c_double_complex), allocatable :: dlnw(:)
   integer :: nbins, nmeas, mnum, warmup, meas_interval, ckpt_loop, loop0
   logical :: resume
   real(c_double) :: glob_dist
@@ -143,5 +147,14 @@ program pqmc_driver
   open (newunit=u_out, file=trim(fout), access='stream', form='unformatted', status='replace')
   write (u_out) acc_all, g_h, fields, meas_sum, int(n_meas, c_int32_t), cnt(1), cnt(2), global_accept, global_reject
   close (u_out)
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, fgrad)
+    allocate (dlnw(nb_ * Lt))
+    call pqmc_check(pqmc_init_g(chain, 1_c_int), 'pqmc_init_g')
+    call pqmc_check(pqmc_gradient(chain, dlnw), 'pqmc_gradient')
+    open (newunit=u_out, file=trim(fgrad), access='stream', form='unformatted', status='replace')
+    write (u_out) dlnw
+    close (u_out)
+  end if
   call pqmc_close()
 end program pqmc_driver

@@ -111,3 +111,30 @@ def test_fortran_driver_matches_oracle(tmp_path, nb, glob, ft):
     assert (got["global_accept"], got["global_reject"]) == (ga, gr)
     if glob:
         assert ga > 0 and ga + gr == 6
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(host.FORTRAN_DRIVER), reason="Fortran driver not built (amdflang missing)")
+def test_fortran_driver_gradient_matches_python_host(tmp_path):
+    """The driver's pqmc_gradient call (iso_c_binding, complex(c_double_complex)
+    array) returns what the Python host gets for the same final fields."""
+    from pqmc_allinone_amd.host import Chain
+    L, Lt, ng, ne, seed = 4, 40, 10, 16, 1234
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne))
+    x, sl = synthetic_inputs(m, seed)
+    fin, fout, fgrad = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "grad.bin"
+    host.write_driver_input(str(fin), m, x, sl, nsweeps=2, seed=seed)
+    r = subprocess.run([host.FORTRAN_DRIVER, str(fin), str(fout), str(tmp_path) + "/", str(fgrad)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = host.read_driver_output(str(fout), m, 2)
+    g_f = np.fromfile(fgrad, np.complex128).reshape(Lt, m.Nb)
+    ch = Chain(m)
+    ch.set_fields(got["fields"])
+    ch.set_trial(sl)
+    ch.init_g(1)
+    g_p = ch.gradient()
+    ch.close()
+    assert np.isfinite(g_f).all() and np.abs(g_f).max() > 0
+    assert np.abs(g_f - g_p).max() <= 1e-12 * np.abs(g_p).max()
+
