@@ -15,7 +15,7 @@
 // GEMM
 
 #define GM_T 32
-#define GM_KC 16
+#define GM_KC 32
 
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zd alpha, const zd *A, int lda,
