@@ -124,3 +124,100 @@ def mixed_expectation(ops, bra, ket, op):
 
 def all_site_subsets(ns, ne):
     return list(itertools.combinations(range(ns), ne))
+
+
+class FockSector:
+    """The fixed-particle-number sector of ns modes with n particles.
+
+    Basis state `mask` is c^dag_{i1} c^dag_{i2} ... |0> with i1 < i2 < ...
+    (the convention of `fock_ops`), so c_i and c^dag_i carry the sign
+    (-1)^(number of occupied modes below i).  Slater determinants have the
+    amplitudes det(R[occ, :]) (ket) and det(L[:, occ]) (bra), each up to one
+    overall constant that cancels in a mixed expectation value."""
+
+    def __init__(self, ns: int, n: int):
+        self.ns, self.n = ns, n
+        masks = np.array([m for m in range(1 << ns) if bin(m).count("1") == n], np.int64)
+        self.masks = masks
+        self.index = -np.ones(1 << ns, np.int64)
+        self.index[masks] = np.arange(masks.size)
+        bits = (masks[:, None] >> np.arange(ns)[None, :]) & 1
+        self.occ = np.array([np.nonzero(b)[0] for b in bits], np.int64)  # (nstates, n), ascending
+        self._hop = {}
+
+    @staticmethod
+    def _sign(mask, i):
+        below = mask & ((1 << i) - 1)
+        cnt = np.zeros_like(below)
+        for k in range(i):
+            cnt += (below >> k) & 1
+        return 1 - 2 * (cnt & 1)
+
+    def _hop_tables(self, a, b):
+        """c^dag_a c_b on this sector: source and destination state indices, signs."""
+        key = (a, b)
+        if key not in self._hop:
+            m = self.masks
+            src = np.nonzero((m >> b) & 1)[0]
+            mb = m[src] & ~np.int64(1 << b)
+            s1 = self._sign(m[src], b)
+            if a != b:
+                ok = ((mb >> a) & 1) == 0
+                src, mb, s1 = src[ok], mb[ok], s1[ok]
+            s2 = self._sign(mb, a)
+            dst = self.index[mb | np.int64(1 << a)]
+            self._hop[key] = (src, dst, (s1 * s2).astype(np.float64))
+        return self._hop[key]
+
+    def hop(self, a, b, v):
+        """(c^dag_a c_b) v for a sector vector v (or a stack v[..., nstates])."""
+        src, dst, sg = self._hop_tables(a, b)
+        out = np.zeros_like(v)
+        out[..., dst] = sg * v[..., src]
+        return out
+
+    def slater_ket(self, R):
+        return np.linalg.det(R[self.occ, :])
+
+    def slater_bra(self, L):
+        return np.linalg.det(np.transpose(L[:, self.occ], (1, 0, 2)))
+
+    def pair_table(self, bra, ket):
+        """T[a*ns + b, c*ns + d] = <L| c^dag_a c_b c^dag_c c_d |R> / <L|R>, in
+        Fock space, for every a, b, c, d.  <L| c^dag_a c_b is the vector
+        (c^dag_b c_a)^T applied to <L| as a column (the operators are real)."""
+        ns = self.ns
+        K = np.empty((ns * ns, ket.size), np.complex128)
+        B = np.empty((ns * ns, bra.size), np.complex128)
+        for a in range(ns):
+            for b in range(ns):
+                K[a * ns + b] = self.hop(a, b, ket)
+                B[a * ns + b] = self.hop(b, a, bra)
+        return (B @ K.T) / (bra @ ket)
+
+    def one_body(self, bra, ket):
+        """<L| c^dag_a c_b |R> / <L|R> as [a, b]."""
+        ns = self.ns
+        out = np.empty((ns, ns), np.complex128)
+        for a in range(ns):
+            for b in range(ns):
+                out[a, b] = bra @ self.hop(a, b, ket)
+        return out / (bra @ ket)
+
+    def pair_annihilation(self, bra, ket, i, j, k, l):
+        """<L| c^dag_i c^dag_j c_k c_l |R> / <L|R> through the sectors n-1 and n-2."""
+        down1 = FockSector(self.ns, self.n - 1)
+        down2 = FockSector(self.ns, self.n - 2)
+
+        def annihilate(sec_from, sec_to, i, v):
+            m = sec_from.masks
+            src = np.nonzero((m >> i) & 1)[0]
+            dst = sec_to.index[m[src] & ~np.int64(1 << i)]
+            out = np.zeros(sec_to.masks.size, np.complex128)
+            out[dst] = sec_from._sign(m[src], i) * v[src]
+            return out
+
+        r = annihilate(down1, down2, k, annihilate(self, down1, l, ket))
+        # <L| c^dag_i c^dag_j = (c_j c_i applied to <L| as a column)
+        lv = annihilate(down1, down2, j, annihilate(self, down1, i, bra))
+        return (lv @ r) / (bra @ ket)

@@ -218,65 +218,161 @@ def test_sweeps_match_dense_products_and_projector_invariants():
     assert np.abs(g @ g - g).max() < 1e-9 and abs(np.trace(g) - ne) < 1e-9
 
 
-def _wick_value(ops, bra, ket, ls, rs, vl, vr):
+# One-body operators of the correlators, restated from src/mod_meas.f90.  An
+# operator is a list of (coefficient, a, b) terms meaning coefficient *
+# c^dag_a c_b, with site 2c the up and 2c+1 the down orbital of cell c.
+
+def _spin_current_terms(i, j, vec):
+    """Spin current of the bond from cell i to cell j: [ci1* cj2, ci2* cj1,
+    cj1* ci2, cj2* ci1] contracted with vec (spin_current_matrix, :64-87)."""
+    ops = [(2 * i, 2 * j + 1), (2 * i + 1, 2 * j), (2 * j, 2 * i + 1), (2 * j + 1, 2 * i)]
+    return [(v, a, b) for v, (a, b) in zip(vec, ops)]
+
+
+def _spin_density_terms(c, vec):
+    """[c1* c1, c1* c2, c2* c1, c2* c2] of cell c (spin_momentum_matrix, :89-108)."""
+    ops = [(2 * c, 2 * c), (2 * c, 2 * c + 1), (2 * c + 1, 2 * c), (2 * c + 1, 2 * c + 1)]
+    return [(v, a, b) for v, (a, b) in zip(vec, ops)]
+
+
+def _kinetic_terms(i, j, vec):
+    """[ci1* cj1, ci2* cj2, cj1* ci1, cj2* ci2] (the last four rows of
+    physical_spin_current_matrix, :129-163)."""
+    ops = [(2 * i, 2 * j), (2 * i + 1, 2 * j + 1), (2 * j, 2 * i), (2 * j + 1, 2 * i + 1)]
+    return [(v, a, b) for v, (a, b) in zip(vec, ops)]
+
+
+def _corr(T, ns, left, right):
+    """<O_left O_right> from the Fock-space table of c^dag c c^dag c products."""
     tot = 0j
-    for i, (l1, l2) in enumerate(ls):
-        for j, (r1, r2) in enumerate(rs):
-            op = ops[l1].T @ ops[l2] @ ops[r1].T @ ops[r2]
-            tot += vl[i] * vr[j] * BF.mixed_expectation(ops, bra, ket, op)
+    for vl, a, b in left:
+        for vr, c, d in right:
+            tot += vl * vr * T[a * ns + b, c * ns + d]
     return tot
 
 
-def test_measurement_matches_fock_space():
-    """L=2 (8 sites): each measured correlator vs the many-body mixed estimator."""
-    L, Lt, ngroup, ne = 2, 4, 2, 4
-    m, x, sl, orc = setup(L=L, Lt=Lt, ngroup=ngroup, ne=ne, seed=3)
+@pytest.mark.parametrize("L1,L2,bc,biased", [(2, 2, (0.0, 0.0), 0.0), (4, 2, (0.5, 0.25), 0.3),
+                                             (2, 4, (0.3, 0.7), 0.0)])
+def test_measurement_matches_fock_space(L1, L2, bc, biased):
+    """Every output slot of take_measurement (src/mod_meas.f90:210-544) against
+    many-body expectation values <L| O_1 O_2 |R> / <L|R> evaluated in the
+    particle-number sector of Fock space (no Wick contraction), with the
+    operators, vectors, boundary phases and displacement bins restated from
+    the reference; the phonon scalars and correlators against numpy.  4 x 2
+    and 2 x 4 lattices tell the +x (+y) neighbour from the -x (-y) one, and
+    nonzero boundary phases exercise add_boundary_phase."""
+    Lt, ngroup, seed, lam, hop = 4, 2, 3, 0.64, 1.0
+    ne = L1 * L2
+    p = ModelParams(L1=L1, L2=L2, Lt=Lt, ngroup=ngroup, ne=ne, lam=lam, bc_phase=bc, biased=biased)
+    m = build_model(p)
+    x, sl = synthetic_inputs(m, seed)
+    orc = O.Oracle(L1, L2, Lt, ngroup, ne, lam=lam, bc_phase=bc, biased=biased)
+    orc.set_fields(x)
+    orc.set_trial(sl)
+    orc.init(1)
     P = trial_q(sl)
     t = Lt // 2
-    # g_h after init sits after slice Lt: R = B(Lt)...B(1) P, L = P^H
+    # after init_g(1), g_h sits after slice Lt: R = B(Lt)...B(1) P, L = P^H
     R = P.copy()
     for tt in range(1, Lt + 1):
         R = BF.slice_matrix(m, x, tt) @ R
     Lm = P.conj().T
     out = orc.measure(t)
-    ops = BF.fock_ops(m.Ns)
-    ket = BF.slater_ket(ops, R)
-    bra = BF.slater_bra(ops, Lm)
-    Nc = m.Ncell
-    off = {n: lo for n, lo, hi in __import__("pqmc_allinone_amd.host", fromlist=["x"]).observable_layout(Nc)}
-    hop = 1.0
+    Ns, Nc = m.Ns, m.Ncell
+    sec = BF.FockSector(Ns, ne)
+    ket, bra = sec.slater_ket(R), sec.slater_bra(Lm)
+    # the amplitudes are those of the determinants: <L|R> = det(L R) (Cauchy-Binet)
+    assert abs(bra @ ket - np.linalg.det(Lm @ R)) < 1e-10 * abs(np.linalg.det(Lm @ R))
+    # g_h(b, a) = <c^dag_a c_b>
+    assert np.abs(sec.one_body(bra, ket).T - orc.g).max() < 1e-10
+    T = sec.pair_table(bra, ket)
+    from pqmc_allinone_amd.host import observable_layout
+    off = {n: lo for n, lo, hi in observable_layout(Nc)}
+    bcp = np.exp(1j * np.pi * np.array(bc))
 
-    def cells(c):
-        x0, y0 = c % L, c // L
-        return ((x0 + 1) % L) + L * y0, x0 + L * ((y0 + 1) % L)
+    def cell(xx, yy):
+        return xx % L1 + L1 * (yy % L2)
 
-    def spin_current(a, b):
-        return [(2 * a, 2 * b + 1), (2 * a + 1, 2 * b), (2 * b, 2 * a + 1), (2 * b + 1, 2 * a)]
+    def phased(vec, c, direction):
+        """add_boundary_phase (:199-208): a cell on the last row (column) of the
+        direction gets the phase on its first two and its conjugate on its
+        last two entries."""
+        xx, yy = c % L1, c // L1
+        last = (xx == L1 - 1) if direction == 0 else (yy == L2 - 1)
+        if not last:
+            return np.array(vec, np.complex128)
+        ph = bcp[direction]
+        return np.array(vec, np.complex128) * np.array([ph, ph, np.conj(ph), np.conj(ph)])
 
-    vjx = -hop * np.array([-1j, -1j, 1j, 1j])
-    vd = np.array([1, 0, 0, 1]) / 2
-    vz = np.array([1, 0, 0, -1]) / 2
-    for d in range(Nc):
-        dx, dy = d % L, d // L
-        den = jxx = sz = sc = 0j
-        for c1 in range(Nc):
-            x1, y1 = c1 % L, c1 // L
-            c2 = (x1 + dx) % L + L * ((y1 + dy) % L)
-            c1x, _ = cells(c1)
-            c2x, _ = cells(c2)
-            sm1 = [(2 * c1, 2 * c1), (2 * c1, 2 * c1 + 1), (2 * c1 + 1, 2 * c1), (2 * c1 + 1, 2 * c1 + 1)]
-            sm2 = [(2 * c2, 2 * c2), (2 * c2, 2 * c2 + 1), (2 * c2 + 1, 2 * c2), (2 * c2 + 1, 2 * c2 + 1)]
-            den += _wick_value(ops, bra, ket, sm1, sm2, vd, vd) / Nc
-            sz += _wick_value(ops, bra, ket, sm1, sm2, vz, vz) / Nc
-            jxx += _wick_value(ops, bra, ket, spin_current(c1, c1x), spin_current(c2, c2x), vjx, vjx) / Nc
+    vjx = -hop * np.array([-1j, -1j, 1j, 1j])    # spin-x current vector
+    vjy = -hop * np.array([-1.0, 1.0, 1.0, -1.0])  # spin-y current vector
+    vd, vz = np.array([1, 0, 0, 1]) / 2, np.array([1, 0, 0, -1]) / 2
+    vx, vy = np.array([0, 1, 1, 0]) / 2, 1j * np.array([0, -1, 1, 0]) / 2
+    names = ["Jxx_Jxx", "Jxy_Jxy", "Jyx_Jyx", "Jyy_Jyy", "Jxx_Jyx", "Jxy_Jyy", "den_den", "SC_SC", "Sz_Sz",
+             "Sx_Sx", "Sy_Sy", "cJyx_cJyx", "cJxy_cJxy", "BFx_BFx", "BFx_BFy", "BFy_BFy", "BFxy_BFxy"]
+    ref = {n: np.zeros(Nc, np.complex128) for n in names}
+    xt = x[t - 1]
+    for c1 in range(Nc):
+        x1, y1 = c1 % L1, c1 // L1
+        c1x, c1y = cell(x1 + 1, y1), cell(x1, y1 + 1)
+        for c2 in range(Nc):
+            x2, y2 = c2 % L1, c2 // L1
+            c2x, c2y = cell(x2 + 1, y2), cell(x2, y2 + 1)
+            d = cell(x2 - x1, y2 - y1)  # get_relative_index: displacement of cell 2 from cell 1
+            # bond currents: x-bond (c, c+x) with the phase of direction 1, y-bond with direction 2
+            ref["Jxx_Jxx"][d] += _corr(T, Ns, _spin_current_terms(c1, c1x, phased(vjx, c1, 0)),
+                                       _spin_current_terms(c2, c2x, phased(vjx, c2, 0))) / Nc
+            ref["Jxy_Jxy"][d] += _corr(T, Ns, _spin_current_terms(c1, c1x, phased(vjy, c1, 0)),
+                                       _spin_current_terms(c2, c2x, phased(vjy, c2, 0))) / Nc
+            ref["Jyx_Jyx"][d] += _corr(T, Ns, _spin_current_terms(c1, c1y, phased(vjx, c1, 1)),
+                                       _spin_current_terms(c2, c2y, phased(vjx, c2, 1))) / Nc
+            # Jyy_Jyy contracts the matrix with vec_l on both sides (:353): the
+            # right operator carries cell 1's phased vector
+            ref["Jyy_Jyy"][d] += _corr(T, Ns, _spin_current_terms(c1, c1y, phased(vjy, c1, 1)),
+                                       _spin_current_terms(c2, c2y, phased(vjy, c1, 1))) / Nc
+            ref["Jxx_Jyx"][d] += _corr(T, Ns, _spin_current_terms(c1, c1x, phased(vjx, c1, 0)),
+                                       _spin_current_terms(c2, c2y, phased(vjx, c2, 1))) / Nc
+            ref["Jxy_Jyy"][d] += _corr(T, Ns, _spin_current_terms(c1, c1x, phased(vjy, c1, 0)),
+                                       _spin_current_terms(c2, c2y, phased(vjy, c2, 1))) / Nc
+            for n, v in (("den_den", vd), ("Sz_Sz", vz), ("Sx_Sx", vx), ("Sy_Sy", vy)):
+                ref[n][d] += _corr(T, Ns, _spin_density_terms(c1, v), _spin_density_terms(c2, v)) / Nc
             # SC_SC = <c1u^dag c1d^dag c2d c2u>
-            op = ops[2 * c1].T @ ops[2 * c1 + 1].T @ ops[2 * c2 + 1] @ ops[2 * c2]
-            sc += BF.mixed_expectation(ops, bra, ket, op) / Nc
-        assert abs(out[off["den_den"] + d] - den) < 1e-10
-        assert abs(out[off["Sz_Sz"] + d] - sz) < 1e-10
-        assert abs(out[off["Jxx_Jxx"] + d] - jxx) < 1e-10
-        assert abs(out[off["SC_SC"] + d] - sc) < 1e-10
-    assert np.all(out[off["Jxx_Jyy"]: off["Jxx_Jyy"] + Nc] == 0)
+            ref["SC_SC"][d] += sec.pair_annihilation(bra, ket, 2 * c1, 2 * c1 + 1, 2 * c2 + 1, 2 * c2) / Nc
+            # physical currents: hop * current vector (no minus sign) plus
+            # ep_parameter * x(bond) * kinetic operator, both phased
+            for n, cc1, cc2, vec, bfo, dirn in (("cJyx_cJyx", c1y, c2y, hop * np.array([-1j, -1j, 1j, 1j]), 1, 1),
+                                                 ("cJxy_cJxy", c1x, c2x, hop * np.array([-1.0, 1, 1, -1]), 0, 0)):
+                kin1 = lam * xt[2 * c1 + bfo] * np.ones(4)
+                kin2 = lam * xt[2 * c2 + bfo] * np.ones(4)
+                left = (_spin_current_terms(c1, cc1, phased(vec, c1, dirn)) +
+                        _kinetic_terms(c1, cc1, phased(kin1, c1, dirn)))
+                right = (_spin_current_terms(c2, cc2, phased(vec, c2, dirn)) +
+                         _kinetic_terms(c2, cc2, phased(kin2, c2, dirn)))
+                ref[n][d] += _corr(T, Ns, left, right) / Nc
+            ref["BFx_BFx"][d] += xt[2 * c1] * xt[2 * c2] / Nc
+            ref["BFx_BFy"][d] += xt[2 * c1] * xt[2 * c2 + 1] / Nc
+            ref["BFy_BFy"][d] += xt[2 * c1 + 1] * xt[2 * c2 + 1] / Nc
+            ref["BFxy_BFxy"][d] += xt[2 * c1] * xt[2 * c1 + 1] * xt[2 * c2] * xt[2 * c2 + 1] / Nc
+    for n in names:
+        got = out[off[n]: off[n] + Nc]
+        scale = max(1.0, np.abs(ref[n]).max())
+        assert np.abs(got - ref[n]).max() < 1e-10 * scale, n
+    # registered but never recorded by the reference (only their Hartree-only
+    # variants are computed, and those are not registered)
+    for n in ("Jxx_Jyy", "Jxy_Jyx"):
+        assert np.all(out[off[n]: off[n] + 2 * Nc] == 0), n
+    # phonon scalars: 0.5 D x^2 - biased x, the time-difference term with the
+    # time+1 neighbour and the 1/(2 delt) * 2 constant, and the mean field
+    xt1 = x[t]
+    dt, M, D = p.dtau, m.M, m.D
+    ke = (0.5 * D * (xt ** 2).sum() - biased * xt.sum()) / Nc
+    pe = (-0.5 * M * ((xt - xt1) ** 2).sum() / dt ** 2 + Nc * 2 / (2 * dt)) / Nc
+    assert abs(out[off["BF_KE"]] - ke) < 1e-12 * max(1, abs(ke))
+    assert abs(out[off["BF_PE"]] - pe) < 1e-12 * max(1, abs(pe))
+    assert abs(out[off["BF_X"]] - xt.sum() / Nc) < 1e-12
+    # every slot is covered: 3 scalars, 19 fields and their k-space twins
+    covered = 3 + (len(names) + 2) * 2 * Nc
+    assert covered == m.total_width
 
 
 def test_kspace_companions_are_fourier_sums():
@@ -286,7 +382,8 @@ def test_kspace_companions_are_fourier_sums():
     lay = {n: (lo, hi) for n, lo, hi in observable_layout(m.Ncell)}
     L = 4
     kx = 2 * np.pi * np.arange(L) / L - np.pi
-    for name in ("den_den", "Jxx_Jxx", "BFx_BFx", "cJxy_cJxy"):
+    from pqmc_allinone_amd.host import FIELDS
+    for name in FIELDS:
         r = out[lay[name][0]: lay[name][1]]
         k = out[lay[name + "_k"][0]: lay[name + "_k"][1]]
         for kk in range(m.Ncell):
