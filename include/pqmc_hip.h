@@ -102,7 +102,8 @@ int pqmc_evolve_g(pqmc_handle *h, int time, int pf_id, int forward);
 int pqmc_evolve_matrix(pqmc_handle *h, int time, int pf_id, int left, int two_way, double *mat, int n);
 
 /* get_g_scratch(time, forward, loop) (src/mod_evolution.f90:113-195).
- * err_fast (may be NULL) receives max|G_fast - G_scratch|. */
+ * err_fast (may be NULL) receives max|G_fast - G_scratch|.  Returns
+ * PQMC_ENUMERIC when the G or ln_cw drift is not finite. */
 int pqmc_g_scratch(pqmc_handle *h, int time, int forward, double *err_fast);
 
 /* Meas_sys%take_measurement(time) + k_space_obs (src/mod_meas.f90:210-566)
@@ -114,7 +115,10 @@ int pqmc_measure(pqmc_handle *h, int time, double *out);
  * get_g_scratch every ngroup slices, and (measure != 0) one measurement at
  * time = Lt/2.  u: 2*Nb*Lt draws in call order.  ntc limits the number of
  * time_count iterations (0 = all).  meas_out (total_width complex) may be
- * NULL.  Accept flags stay on the device (pqmc_get_accepts). */
+ * NULL.  Accept flags stay on the device (pqmc_get_accepts).  The drift of
+ * every stabilization of the sweep is kept on the device and read once at
+ * the end: PQMC_ENUMERIC if any of them was not finite (the sweep's later
+ * results are then invalid). */
 int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, int measure, double *meas_out);
 
 /* n chains, one sweep each, advanced at the same time (the batched form):
@@ -148,8 +152,21 @@ int pqmc_get_dstring(pqmc_handle *h, double *d);              /* D_string(ne, nb
  * since then (src/mod_update.f90:142, src/mod_evolution.f90:330-357). */
 int pqmc_get_lncw(pqmc_handle *h, double *re_im);
 int pqmc_get_accepts(pqmc_handle *h, uint8_t *acc, int64_t n); /* last sweep, call order */
-/* positive_accept, negative_accept, sync timeouts; err_fast_max. */
+/* positive_accept, negative_accept, sync timeouts; err_fast_max, the running
+ * maximum of both drift checks of get_g_scratch_T0 (max|G_fast - G_scratch|
+ * and |Re(ln_cw - ln_cw_fast)|, src/mod_evolution.f90:179-193) over every
+ * stabilization, starting from 1e-6 as the reference's module variable. */
 int pqmc_get_counters(pqmc_handle *h, int64_t *out3, double *err_fast_max);
+/* out2: the largest G drift and the largest Re ln_cw drift over the
+ * stabilizations of the latest pqmc_sweep, pqmc_gradient or pqmc_g_scratch
+ * call (not floored at 1e-6).  Diagnostics; the reference prints each new
+ * maximum instead (src/mod_evolution.f90:180-192). */
+int pqmc_get_drift(pqmc_handle *h, double *out2);
+/* The same two drifts for each stabilization of the latest pqmc_sweep,
+ * pqmc_gradient or pqmc_g_scratch call, in order: out[2 k], out[2 k + 1] for
+ * k < min(cap, *n); *n receives the number of stabilizations.  Diagnostics,
+ * as the reference's per-event prints (src/mod_evolution.f90:180-192). */
+int pqmc_get_drift_history(pqmc_handle *h, double *out, int32_t cap, int32_t *n);
 /* Sets positive_accept, negative_accept (in2) and err_fast_max, for a run
  * resumed from a checkpoint.  No reference counterpart: the reference keeps
  * these as module globals (src/mod_update.f90, src/Main_PQMC.f90:222-224)

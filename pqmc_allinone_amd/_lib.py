@@ -68,6 +68,8 @@ SIGNATURES = {
     "pqmc_gradient": (_I, [_P, _P]),
     "pqmc_get_accepts": (_I, [_P, _P, C.c_int64]),
     "pqmc_get_counters": (_I, [_P, _P, _P]),
+    "pqmc_get_drift": (_I, [_P, _P]),
+    "pqmc_get_drift_history": (_I, [_P, _P, C.c_int32, _P]),
     "pqmc_set_counters": (_I, [_P, _P, C.c_double]),
     "pqmc_synchronize": (_I, [_P]),
     "pqmc_ran2_seed": (None, [C.POINTER(Ran2), C.c_int32]),

@@ -637,8 +637,17 @@ extern "C" hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s)
 #define SP_TWO_PI 6.28318548202514648438  // 2*PI in REAL(SP), src/mod_nrtype.f90:42
 #define DP_PI 3.14159265358979323846
 
+// With drift_call set, the fast log-weight, the value *lncw_out held before
+// plus the accepts' log(real(R_elec)) terms *acc since then, is compared with
+// the new one: |Re(new - fast)| goes into *drift_call and (as a running
+// maximum) *drift_run, uint64 bits as in k_maxdiff (get_g_scratch_T0,
+// src/mod_evolution.f90:187-192).  An accepted ratio with a negative real
+// part makes the fast value NaN (log of a negative real,
+// src/mod_update.f90:142); the reference's comparison then fails and the
+// check is skipped, and so it is here.  A non-finite new value counts as +inf.
 __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
-                       int tr_weight, int ncopy, zd *lncw_out) {
+                       int tr_weight, int ncopy, zd *lncw_out, const double *acc, unsigned long long *drift_call,
+                       unsigned long long *drift_run) {
   __shared__ double sr[4], si[4];
   const int tid = threadIdx.x;
   double ar = 0.0, ai = 0.0;
@@ -669,13 +678,23 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
     if (tr_weight) lw = zadd(lw, zconj(lw));
     lw = zscale(lw, (double)ncopy);
     lw.y = fmod(lw.y, SP_TWO_PI);
+    if (drift_call) {
+      const double fast = lncw_out->x + *acc;
+      double e = isfinite(lw.x) ? fabs(lw.x - fast) : INFINITY;
+      if (e == e) {
+        atomicMax(drift_call, (unsigned long long)__double_as_longlong(e));
+        atomicMax(drift_run, (unsigned long long)__double_as_longlong(e));
+      }
+    }
     *lncw_out = lw;
   }
 }
 
 extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
-                                int tr_weight, int ncopy, zd *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, dlog, nblk, tr_weight, ncopy, out);
+                                int tr_weight, int ncopy, zd *out, const double *acc, unsigned long long *drift_call,
+                                unsigned long long *drift_run, hipStream_t s) {
+  hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, dlog, nblk, tr_weight, ncopy, out, acc,
+                     drift_call, drift_run);
   return hipGetLastError();
 }
 
@@ -894,8 +913,10 @@ extern "C" hipError_t pqmc_ft_finish(zd *G, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
-// max |a - b| over n complex entries into *out (as uint64 bits of a nonnegative double)
-__global__ void k_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out) {
+// max |a - b| over n complex entries into *out, and into *run when set (as
+// uint64 bits of a nonnegative double, so that an unsigned atomic max is the
+// max of the values; a NaN is taken as +inf)
+__global__ void k_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, unsigned long long *run) {
   double m = 0.0;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     const zd d = zsub(a[e], b[e]);
@@ -913,14 +934,16 @@ __global__ void k_maxdiff(const zd *a, const zd *b, long n, unsigned long long *
       if (sm[w] > m) m = sm[w];
     }
     atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if (run) atomicMax(run, (unsigned long long)__double_as_longlong(m));
   }
 }
 
-extern "C" hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s) {
+extern "C" hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, unsigned long long *run,
+                                   hipStream_t s) {
   int blocks = (int)((n + 1023) / 1024);
   if (blocks > 256) blocks = 256;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_maxdiff, dim3(blocks), dim3(256), 0, s, a, b, n, out);
+  hipLaunchKernelGGL(k_maxdiff, dim3(blocks), dim3(256), 0, s, a, b, n, out, run);
   return hipGetLastError();
 }
 

@@ -1,6 +1,8 @@
 // Shared declarations of the fused family-pass kernel (k_pass.hip) and its
 // host launcher.
 #pragma once
+#include <cstddef>
+
 #include "pqmc_common.h"
 
 #define PS_THREADS 512
@@ -13,11 +15,18 @@
 // blocks) of B = A T^-1, with
 //     G_after[:, q] = G_before[:, q] + G_before[:, S_A] B w[q],  w[q] = (I - G_before)[S_A, q].
 // The accept count itself travels in the block's flag word.
-struct ps_rec {
+// Helpers read a record with plain, L2-cached loads after its flag, so no
+// 128-byte line of one record may also hold part of another: a line cached
+// while reading record k would otherwise serve stale indices of record k+1.
+// The header fills one whole line and the record is a whole number of lines
+// (the array itself comes from hipMalloc, 256-byte aligned).
+struct __align__(128) ps_rec {
   int li[PS_NB];          // block-local indices of the accepted bonds, ascending
-  int pad[8];
+  int pad[32 - PS_NB];
   zd AT[PS_BS * PS_BS];   // A on and above the diagonal blocks, T^-1 below; row-major, stride 4a
 };
+static_assert(sizeof(ps_rec) % 128 == 0, "a decision record must be a whole number of 128-byte lines");
+static_assert(offsetof(ps_rec, AT) == 128, "the record header must fill exactly one 128-byte line");
 
 struct ps_args {
   model_t md;
@@ -43,7 +52,10 @@ struct ps_args {
   unsigned long long *prof;   // optional [gridDim][8] wall-clock ticks
 };
 
-// column stride and size of the slot
+// column stride and size of the slot: whole 128-byte lines (8 complex
+// entries), like the tile-slot columns of 2 PS_BS entries, so that every
+// column read with plain loads starts and ends on a line boundary
+static_assert((2 * PS_BS * sizeof(zd)) % 128 == 0, "tile-slot columns must be whole 128-byte lines");
 static inline int pqmc_pass_slot_ld(int Ns) { return (Ns + 7) & ~7; }
 static inline size_t pqmc_pass_slot_elems(int Ns, int nblk) {
   return (size_t)(nblk + 1) * 2 * PS_BS * pqmc_pass_slot_ld(Ns);

@@ -189,6 +189,32 @@ static __device__ __forceinline__ double row4_sum(double v) {
   return v;
 }
 
+// max over the 16 lanes of each 16-lane row
+static __device__ __forceinline__ double row16_max(double v) {
+  v = fmax(v, dpp_move<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_move<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_move<0x141, 0xF>(v));  // row_half_mirror
+  v = fmax(v, dpp_move<0x140, 0xF>(v));  // row_mirror
+  return v;
+}
+
+// value of lane l (uniform l) in every lane
+static __device__ __forceinline__ double rdlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+static __device__ __forceinline__ zd rdlane_z(zd v, int l) { return zmk(rdlane_d(v.x, l), rdlane_d(v.y, l)); }
+static __device__ __forceinline__ zd shfl_z(zd v, int l) { return zmk(__shfl(v.x, l, 64), __shfl(v.y, l, 64)); }
+
+// The adjugate gives M^-1 = C^T / det accurately only while the cofactor
+// expansion does not cancel: its error grows like max|M|^4 / |det| (in units
+// of the rounding), and after an ill-conditioned accept (a small ratio that
+// the bosonic action still lets through) G briefly holds entries of 1e5 and
+// more, where it would lose most digits.  Beyond this bound the inverse is
+// taken by Gauss-Jordan elimination with partial pivoting instead, as the
+// reference inverts R with LU (inverse, src/mod_matrixlib.f90:410-431).
+#define PS_ADJ_LIMIT 1.0e4
+
 // modi of src/mod_matrixlib.f90:66-77
 static __device__ __forceinline__ int modi_d(int n, int p) {
   int r = n % p;
@@ -561,6 +587,40 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         if (gq == f) L.sCof[ll] = cof;
         // X = Delta M^-1 with M^-1(kk, c) = C(c, kk) / det
         wave_sync();
+        {
+          const zd me = L.sM[gq][ll];
+          const double nrm = rdlane_d(row16_max(fabs(me.x) + fabs(me.y)), 16 * f);
+          if (nrm * nrm * nrm * nrm > PS_ADJ_LIMIT * sqrt(n2)) {
+            // Gauss-Jordan on [M | I] with partial pivoting (largest |re| + |im|,
+            // the first of equals, as izamax in zgetrf); lane l < 32 holds entry
+            // (l & 3, l >> 2) of the 4 x 8 array, lanes 32..63 idle along
+            const int gr = lane & 3, gc = lane >> 2;
+            zd a = zmk(0.0, 0.0);
+            if (lane < 32) a = gc < 4 ? L.sM[f][gr + 4 * gc] : zmk(gr == gc - 4 ? 1.0 : 0.0, 0.0);
+            for (int k = 0; k < 4; ++k) {
+              int pr = k;
+              double best = -1.0;
+              for (int i = k; i < 4; ++i) {
+                const zd v = rdlane_z(a, i + 4 * k);
+                const double sv = fabs(v.x) + fabs(v.y);
+                if (sv > best) {
+                  best = sv;
+                  pr = i;
+                }
+              }
+              a = shfl_z(a, (gr == k ? pr : (gr == pr ? k : gr)) + 4 * gc);
+              const zd piv = rdlane_z(a, 5 * k);
+              const double pn = piv.x * piv.x + piv.y * piv.y;
+              if (gr == k) a = zmul(a, zmk(piv.x / pn, -piv.y / pn));
+              const zd rk = shfl_z(a, k + 4 * gc);  // row k, scaled
+              const zd fk = shfl_z(a, gr + 4 * k);  // this row at column k
+              if (gr != k) a = zsub(a, zmul(fk, rk));
+            }
+            // det M^-1(kk, c) in the place of the cofactor C(c, kk)
+            if (lane < 32 && gc >= 4) L.sCof[(gc - 4) + 4 * gr] = zmul(zmk(dre, dim), a);
+            wave_sync();
+          }
+        }
         if (lane < 16) {
           zd dv[4], cv[4];
 #pragma unroll

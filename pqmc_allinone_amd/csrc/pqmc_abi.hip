@@ -43,10 +43,12 @@ hipError_t pqmc_eye(zd *Q, int ldq, int m, int n, hipStream_t s);
 hipError_t pqmc_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int conjt, hipStream_t s);
 hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, zd *work, hipStream_t s);
 hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
-                     int tr_weight, int ncopy, zd *out, hipStream_t s);
+                     int tr_weight, int ncopy, zd *out, const double *acc, unsigned long long *drift_call,
+                     unsigned long long *drift_run, hipStream_t s);
 hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s);
-hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, hipStream_t s);
+hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, unsigned long long *run,
+                        hipStream_t s);
 hipError_t pqmc_conjt(const zd *A, int lda, int m, int n, zd *C, int ldc, hipStream_t s);
 hipError_t pqmc_mfma_probe(const double *A, const double *B, double *C, hipStream_t s);
 hipError_t pqmc_launch_measure(const model_t *md, const zd *G, const double *x, int time, const zd *phase,
@@ -121,7 +123,17 @@ struct pqmc_handle {
   double *d_lncw_acc = nullptr;
   zd *d_grad = nullptr;  // field gradient, [Lt][Nb]
   zd *d_lncw = nullptr;
-  unsigned long long *d_stats = nullptr, *d_err = nullptr;
+  unsigned long long *d_stats = nullptr;
+  // drift of the stabilizations, uint64 bits of nonnegative doubles (NaN as
+  // +inf): [0] max|G_fast - G_scratch| and [1] |Re(ln_cw - ln_cw_fast)| of
+  // the latest get_g_scratch, [2] and [3] their running maxima over every
+  // stabilization since the host last read them
+  unsigned long long *d_err = nullptr;
+  double last_drift[2] = {0.0, 0.0};  // [2] and [3] as of the latest read
+  // [0] and [1] of every stabilization of the latest sweep, gradient pass or
+  // get_g_scratch call, in order (room for Lt / ngroup + 1)
+  unsigned long long *d_hist = nullptr;
+  int nhist = 0, hist_cap = 0;
   zd *d_meas = nullptr, *d_kphase = nullptr;
   unsigned long long *d_prof = nullptr;  // family-pass phase timers (debug)
   // global-move weight (allocated on first use): proposed fields and their exponentials
@@ -289,7 +301,9 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_lncw_acc, 1);
   A_(h->d_lncw, 1);
   A_(h->d_stats, 4);
-  A_(h->d_err, 1);
+  A_(h->d_err, 4);
+  h->hist_cap = h->Lt / h->ngroup + 1;
+  A_(h->d_hist, 2 * (size_t)h->hist_cap);
   A_(h->d_meas, h->total_width);
   A_(h->d_kphase, (size_t)h->Ncell * h->Ncell);
   if (h->ft) {
@@ -313,6 +327,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
       hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
+      hipMemsetAsync(h->d_err, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_acc, 0, (size_t)h->Nb * h->Lt, h->st) != hipSuccess) {
     pqmc_destroy(h);
     return fail(PQMC_EHIP, "initial memset failed");
@@ -347,7 +362,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->d_Q, h->d_D, h->d_R, h->d_dlog, h->d_rinv, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
-                  h->d_lncw, h->d_stats, h->d_err, h->d_meas, h->d_kphase, h->d_prof,
+                  h->d_lncw, h->d_stats, h->d_err, h->d_hist, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
                   h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad};
   for (void *p : ptrs)
@@ -455,7 +470,9 @@ static int run_qr(pqmc_handle *h, qr_ws &w, zd *A, int lda, int m, int n, zd *Q,
 
 // G = QR (QL^H QR)^-1 QL^H with ln_cw (cal_g_and_det_T0, src/mod_evolution.f90:330-357).
 // QL is the left B-string stored as lmat^H (Ns x ne); QR is the right one.
-static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout) {
+// drift: a stabilization (get_g_scratch), whose ln_cw is compared with the
+// fast one into the drift words.
+static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout, bool drift) {
   const int Ns = h->Ns, ne = h->ne;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   // M = QL^H QR
@@ -469,7 +486,7 @@ static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout) {
   // G = QR Z
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRm, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
   HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_dlog, h->nblock, h->md.tr_weight, h->md.ncopy,
-                   h->d_lncw, h->st));
+                   h->d_lncw, h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, h->st));
   // the accepts' log(real(R_elec)) terms since the last stabilization are
   // overwritten with it (src/mod_update.f90:142)
   HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
@@ -615,7 +632,7 @@ static int ft_left(pqmc_handle *h, int b, int t0, int t1) {
 // a QR (explicit Q into QM unless null), and
 // ln det(1 + B) = ln det M + ln det Q_r + ln det Q_l + sum ln Db_r + sum ln Db_l
 // into *lncw (cal_g_and_det_FT, src/mod_evolution.f90:494-648)
-static int ft_weight(pqmc_handle *h, ft_udt r, ft_udt l, zd *QM, zd *lncw) {
+static int ft_weight(pqmc_handle *h, ft_udt r, ft_udt l, zd *QM, zd *lncw, bool drift = false) {
   const int Ns = h->Ns;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   zd *QQ = FW(h, 0), *TT = FW(h, 1), *M = FW(h, 2);
@@ -625,18 +642,19 @@ static int ft_weight(pqmc_handle *h, ft_udt r, ft_udt l, zd *QM, zd *lncw) {
   HIPCHK(pqmc_ft_dlog(r.D, l.D, r.L, l.L, Ns, h->d_flog, h->st));
   int rc = run_qr(h, h->qf, M, Ns, Ns, Ns, QM, nullptr, nullptr, 0);
   if (rc) return rc;
-  HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, lncw, h->st));
+  HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, lncw,
+                   h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, h->st));
   return PQMC_OK;
 }
 
 // G = (1 + Q_r D_r T_r T_l D_l Q_l)^-1 = Q_l^H Db_l^-1 M^-1 Db_r^-1 Q_r^H;
-// Gout = 1 - G; ln_cw = ln det(1 + B)
-static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout) {
+// Gout = 1 - G; ln_cw = ln det(1 + B); drift as in cal_g
+static int ft_cal_g(pqmc_handle *h, int br, int bl, zd *Gout, bool drift) {
   const int Ns = h->Ns;
   const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   zd *TT = FW(h, 1), *M = FW(h, 2), *QM = FW(h, 3), *Z = FW(h, 4);
   const zd *Qr = FQS(h, br), *Ul = FQS(h, bl), *Dr = FDS(h, br), *Dl = FDS(h, bl);
-  int rc = ft_weight(h, ft_block(h, br), ft_block(h, bl), QM, h->d_lncw);
+  int rc = ft_weight(h, ft_block(h, br), ft_block(h, bl), QM, h->d_lncw, drift);
   if (rc) return rc;
   // TT <- Db_r^-1 Q_r^H;  Z = R_M^-1 Q_M^H TT;  Z <- Db_l^-1 Z;  G = Q_l^H Z
   HIPCHK(pqmc_conjt(Qr, Ns, Ns, Ns, TT, Ns, h->st));
@@ -657,7 +675,7 @@ static int ft_init(pqmc_handle *h) {
   int rc = ft_identity(h, 1);
   if (!rc) rc = ft_identity(h, nb);
   for (int b = 2; b <= nb - 1 && !rc; ++b) rc = ft_right(h, b, (b - 2) * ng + 1, (b - 1) * ng);
-  if (!rc) rc = ft_cal_g(h, nb - 1, nb, h->d_G);
+  if (!rc) rc = ft_cal_g(h, nb - 1, nb, h->d_G, false);
   return rc;
 }
 
@@ -668,13 +686,13 @@ static int ft_scratch(pqmc_handle *h, int time, int forward) {
     const int block = time / ng + 1;
     if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
     int rc = ft_right(h, block, time - ng, time - 1);
-    return rc ? rc : ft_cal_g(h, block, block + 1, h->d_Gn);
+    return rc ? rc : ft_cal_g(h, block, block + 1, h->d_Gn, true);
   }
   int block = time / ng + 2;
   if (ng == 1) block -= 1;
   if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
   int rc = ft_left(h, block, time, time + ng - 1);
-  return rc ? rc : ft_cal_g(h, block - 1, block, h->d_Gn);
+  return rc ? rc : ft_cal_g(h, block - 1, block, h->d_Gn, true);
 }
 
 extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
@@ -721,7 +739,7 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
     }
     p = pend + 1;
   }
-  int rc = cal_g(h, QS(h, nb), rmat, h->d_G);
+  int rc = cal_g(h, QS(h, nb), rmat, h->d_G, false);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   h->g_ready = true;
@@ -857,6 +875,8 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
   const int Ns = h->Ns, ne = h->ne, ng = h->ngroup;
   int rc = PQMC_OK;
   span_guard sg(h, 2);
+  // this stabilization's drift words; the running maxima stay
+  HIPCHK(hipMemsetAsync(h->d_err, 0, 2 * sizeof(unsigned long long), h->st));
   const size_t qb = sizeof(zd) * Ns * ne;
   const zd *lmatH = nullptr, *rmat = nullptr;
   if (h->ft) {
@@ -887,30 +907,44 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     rmat = QS(h, block - 1);
   }
   if (!h->ft) {
-    rc = cal_g(h, lmatH, rmat, h->d_Gn);
+    rc = cal_g(h, lmatH, rmat, h->d_Gn, true);
     if (rc) return rc;
   }
-  HIPCHK(hipMemsetAsync(h->d_err, 0, sizeof(unsigned long long), h->st));
-  HIPCHK(pqmc_maxdiff(h->d_G, h->d_Gn, (long)Ns * Ns, h->d_err, h->st));
+  HIPCHK(pqmc_maxdiff(h->d_G, h->d_Gn, (long)Ns * Ns, h->d_err, h->d_err + 2, h->st));
+  if (h->nhist < h->hist_cap)
+    HIPCHK(hipMemcpyAsync(h->d_hist + 2 * h->nhist++, h->d_err, 2 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToDevice, h->st));
   std::swap(h->d_G, h->d_Gn);
   return PQMC_OK;
 }
 
+// Reads the drift words once (after a sweep, a gradient pass or one
+// get_g_scratch): every stabilization since the previous read is in the
+// running maxima, which are cleared for the next ones.  err_fast_max keeps
+// the largest G or Re ln_cw drift, as get_g_scratch_T0 does
+// (src/mod_evolution.f90:179-193); a non-finite one fails the call.
 static int read_err(pqmc_handle *h, double *err) {
-  unsigned long long bits = 0;
-  HIPCHK(hipMemcpyAsync(&bits, h->d_err, sizeof(bits), hipMemcpyDeviceToHost, h->st));
+  unsigned long long bits[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(bits, h->d_err, sizeof(bits), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemsetAsync(h->d_err + 2, 0, 2 * sizeof(unsigned long long), h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  double e;
-  std::memcpy(&e, &bits, sizeof(e));
-  if (err) *err = e;
-  if (e > h->err_fast_max) h->err_fast_max = e;
-  if (!std::isfinite(e)) return fail(PQMC_ENUMERIC, "non-finite Green's function after stabilization");
+  double e[4];
+  std::memcpy(e, bits, sizeof(e));
+  if (err) *err = e[0];
+  h->last_drift[0] = e[2];
+  h->last_drift[1] = e[3];
+  for (int i = 2; i < 4; ++i)
+    if (e[i] > h->err_fast_max) h->err_fast_max = e[i];
+  if (!std::isfinite(e[2]))
+    return fail(PQMC_ENUMERIC, "non-finite Green's function at a stabilization");
+  if (!std::isfinite(e[3])) return fail(PQMC_ENUMERIC, "non-finite log-weight at a stabilization");
   return PQMC_OK;
 }
 
 extern "C" int pqmc_g_scratch(pqmc_handle *h, int time, int forward, double *err_fast) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
   if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  h->nhist = 0;
   int rc = g_scratch(h, time, forward);
   if (rc) return rc;
   return read_err(h, err_fast);
@@ -938,6 +972,7 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   HIPCHK(hipMemcpyAsync(h->d_u, u, sizeof(float) * nu, hipMemcpyHostToDevice, h->st));
   size_t off = 0;
   bool measured = false;
+  h->nhist = 0;
   for (int tc = 1; tc <= ntc; ++tc) {
     int time;
     if (!forward) {
@@ -993,6 +1028,7 @@ extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
   if (!h->d_grad) HIPCHK(dalloc(&h->d_grad, n));
   HIPCHK(hipMemsetAsync(h->d_grad, 0, sizeof(zd) * n, h->st));
   int rc = PQMC_OK;
+  h->nhist = 0;
   for (int tc = 1; tc <= Lt; ++tc) {
     const int time = Lt - tc + 1;
     for (int f = 3; f >= 0; --f) {
@@ -1272,7 +1308,7 @@ extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
   rc = run_qr(h, h->qm, h->d_M, ne, ne, ne, nullptr, nullptr, nullptr, 0);
   if (rc) return rc;
   HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_glog, nlog, h->md.tr_weight, h->md.ncopy, h->d_gw,
-                   h->st));
+                   nullptr, nullptr, nullptr, h->st));
   zd v;
   HIPCHK(hipMemcpyAsync(&v, h->d_gw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -1298,6 +1334,24 @@ extern "C" int pqmc_get_counters(pqmc_handle *h, int64_t *o3, double *efm) {
   o3[1] = (int64_t)s[1];
   o3[2] = (int64_t)s[2];
   if (efm) *efm = h->err_fast_max;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_drift(pqmc_handle *h, double *o2) {
+  if (!h || !o2) return fail(PQMC_EINVAL, "null argument");
+  o2[0] = h->last_drift[0];
+  o2[1] = h->last_drift[1];
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_get_drift_history(pqmc_handle *h, double *out, int32_t cap, int32_t *n) {
+  if (!h || !n || (cap > 0 && !out)) return fail(PQMC_EINVAL, "null argument");
+  *n = h->nhist;
+  const int k = std::min(cap, h->nhist);
+  if (k > 0) {
+    HIPCHK(hipMemcpyAsync(out, h->d_hist, 2 * sizeof(double) * k, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
   return PQMC_OK;
 }
 

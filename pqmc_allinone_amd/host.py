@@ -313,6 +313,23 @@ class Chain:
         return {"positive_accept": int(o[0]), "negative_accept": int(o[1]), "sync_timeouts": int(o[2]),
                 "err_fast_max": efm.value}
 
+    def drift(self):
+        """Largest G and Re ln_cw drift over the stabilizations of the latest
+        sweep, gradient pass or get_g_scratch (max|G_fast - G_scratch|,
+        |Re(ln_cw - ln_cw_fast)|, src/mod_evolution.f90:179-193)."""
+        o = np.zeros(2)
+        _lib.check(self._L.pqmc_get_drift(self.h, _lib.ptr(o)))
+        return float(o[0]), float(o[1])
+
+    def drift_history(self) -> np.ndarray:
+        """(n, 2): G and Re ln_cw drift of each stabilization of the latest
+        sweep, gradient pass or get_g_scratch, in order."""
+        cap = self.model.p.Lt // self.model.p.ngroup + 1
+        o = np.zeros((cap, 2))
+        n = C.c_int32(0)
+        _lib.check(self._L.pqmc_get_drift_history(self.h, _lib.ptr(o), cap, C.addressof(n)))
+        return o[: min(n.value, cap)].copy()
+
     def set_counters(self, positive_accept: int, negative_accept: int, err_fast_max: float):
         """Restore the accept counters and err_fast_max (resume from a checkpoint)."""
         v = np.array([positive_accept, negative_accept], np.int64)

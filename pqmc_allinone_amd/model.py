@@ -190,3 +190,17 @@ def synthetic_inputs(m: Model, seed: int):
 def sweep_draw_count(m: Model) -> int:
     """ran2 draws per sweep: two per bond per slice (src/mod_update.f90:84,117)."""
     return 2 * m.Nb * m.p.Lt
+
+
+# The mixed batch of independent chains: 32 chains at each of L = 8, 10, 12,
+# 14 (Ns = 2 L^2, ne = L^2), Lt cycling through 400, 600, 800, 1000 within
+# each lattice, fields, trial perturbation and draws from seed 1000 + i.
+MIXED_LATTICES = (8, 10, 12, 14)
+MIXED_LTS = (400, 600, 800, 1000)
+
+
+def mixed_chain_specs(n: int = 128, seed0: int = 1000):
+    """(L, Lt, seed) of chain i of the mixed batch, i = 0..n-1."""
+    per = max(1, n // len(MIXED_LATTICES))
+    return [(MIXED_LATTICES[min(i // per, len(MIXED_LATTICES) - 1)], MIXED_LTS[i % len(MIXED_LTS)], seed0 + i)
+            for i in range(n)]

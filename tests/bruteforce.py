@@ -221,3 +221,33 @@ class FockSector:
         # <L| c^dag_i c^dag_j = (c_j c_i applied to <L| as a column)
         lv = annihilate(down1, down2, j, annihilate(self, down1, i, bra))
         return (lv @ r) / (bra @ ket)
+
+
+def inv_extended(M):
+    """Inverse by Gauss-Jordan with partial pivoting in numpy's extended
+    precision (clongdouble): a reference for ill-conditioned 4x4 blocks."""
+    n = M.shape[0]
+    a = np.concatenate([M.astype(np.clongdouble), np.eye(n, dtype=np.clongdouble)], 1)
+    for i in range(n):
+        p = i + int(np.argmax(np.abs(a[i:, i])))
+        a[[i, p]] = a[[p, i]]
+        a[i] /= a[i, i]
+        for k in range(n):
+            if k != i:
+                a[k] -= a[k, i] * a[i]
+    return a[:, n:]
+
+
+def sequential_updates_extended(G, deltas):
+    """G after the rank-4 updates G += G[:, S] X (I - G)[S, :],
+    X = Delta (I + G[S, S] Delta)^-1, applied in order (update_phonon_field,
+    src/mod_update.f90:97-190), in extended precision.  deltas: list of
+    (sites S, Delta)."""
+    ns = G.shape[0]
+    g = G.astype(np.clongdouble)
+    eye = np.eye(ns, dtype=np.clongdouble)
+    for S, D in deltas:
+        Dl = D.astype(np.clongdouble)
+        M = np.eye(4, dtype=np.clongdouble) + g[np.ix_(S, S)] @ Dl
+        g = g + g[:, S] @ (Dl @ inv_extended(M)) @ (eye - g)[S, :]
+    return g.astype(np.complex128)

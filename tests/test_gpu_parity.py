@@ -140,11 +140,16 @@ def test_sweeps_match_oracle(nb):
         d = abs(lg.imag - lo.imag) % (2 * np.pi)
         nterms = m.p.ne * m.nblock + 1
         assert min(d, 2 * np.pi - d) < 1e-9 + nterms / 2 * 1.8e-7, (sweep, lg, lo)
+        # the largest G and Re ln_cw drift over the sweep's 10 stabilizations
+        dg, dl = ch.drift()
+        assert 0.0 < dg < 1e-8 and 0.0 <= dl < 1e-8 * max(1.0, abs(lo.real)), (sweep, dg, dl)
     assert np.array_equal(ch.get_fields(), orc.get_fields())
     c = ch.counters()
     assert c["sync_timeouts"] == 0
-    pa, na, _ = orc.counters()
+    pa, na, efm = orc.counters()
     assert (c["positive_accept"], c["negative_accept"]) == (pa, na)
+    # running maximum of both drift checks, from the reference's initial 1e-6
+    assert abs(c["err_fast_max"] - efm) < 1e-9, (c["err_fast_max"], efm)
 
 
 @pytest.mark.parametrize("L,Lt,ntc,nb", [(8, 40, 20, 0), (8, 40, 20, 5), (14, 40, 10, 0), (20, 40, 20, 0)])
