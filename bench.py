@@ -13,10 +13,16 @@ advanced together (pqmc_sweep_batch), their aggregate chain-sweeps/s and the
 largest |G - G_plain| of the batch's first chain, which replays the plain
 chain's fields and draws.
 
+`mixed_batch` reports 128 independent chains of uneven lattice and Lt (32
+each at L = 8, 10, 12, 14; Lt cycling through 400..1000; seeds 1000 + i)
+advanced by pqmc_sweep_many, 16 at a time.
+
 Prints one JSON line.  `roofline` is that of the dominant kernel class,
 timed with HIP events on the chain's own stream over the timed region;
-`cpu_baseline` times the oracle (a single-threaded C restatement of the
-reference algorithm) on a bounded slice of the same workload.
+`cpu_baseline` runs the oracle (a single-threaded C restatement of the
+reference algorithm) as the reference deploys itself: one chain per host
+core on all cores of this process's share at once, each over one full sweep
+of the workload; `value` there is the mean per-core rate.
 """
 from __future__ import annotations
 
@@ -41,14 +47,23 @@ CONFIGS = {
 }
 
 
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def sweep_flops(Ns, ne, Nb, Lt, ng, n_acc, d=4):
     """F_sweep of the survey's accounting: rank-4 updates + wraps + propagation + stabilization."""
     return (8 * d * Ns ** 2 * n_acc + Lt * Nb * (16 * d * d * Ns + 8 * d * d * ne)
             + (Lt / ng) * (32 * Ns * ne ** 2 + (16 / 3) * ne ** 3 + 8 * Ns ** 2 * ne))
 
 
-def cpu_baseline(L, Lt, ng, ne, seed, target_s):
-    """The oracle on the first slices of a backward sweep, scaled to chain-sweeps/s."""
+def _cpu_chain(spec):
+    """One oracle chain (a process of its own): init_g, then one full backward
+    sweep with the measurement at Lt/2, timed without the init."""
+    L, Lt, ng, ne, seed = spec
+    t_start = time.time()
+    sys.path.insert(0, ROOT)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
     from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
@@ -58,24 +73,105 @@ def cpu_baseline(L, Lt, ng, ne, seed, target_s):
     orc.set_fields(x)
     orc.set_trial(sl)
     orc.init(1)
-    rng = O.Ran2Stream(seed)
-    u = rng.draws(2 * m.Nb * Lt)
-    # calibrate on one stabilization group, then run a bounded sample
+    u = O.Ran2Stream(seed).draws(2 * m.Nb * Lt)
+    t_init = time.time()
     t0 = time.perf_counter()
-    orc.sweep(False, u, ntc=ng)
-    t_group = time.perf_counter() - t0
-    ngroups = max(1, min(Lt // ng - 1, int(target_s / max(t_group, 1e-9))))
-    orc2 = O.Oracle(L, L, Lt, ng, ne)
-    orc2.set_fields(x)
-    orc2.set_trial(sl)
-    orc2.init(1)
+    orc.sweep(False, u, measure=True)
+    return time.perf_counter() - t0, t_start, t_init
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(L, Lt, ng, ne, seed, cores):
+    """The oracle as the reference deploys itself, one independent chain per
+    core on `cores` host cores at once (one MPI rank per core), each timed
+    over one full sweep of the workload; seeds seed + i."""
+    import multiprocessing as mp
+    # one thread per chain, BLAS included, as the reference's runs set
+    # OPENBLAS_NUM_THREADS / MKL_NUM_THREADS = 1 per rank (submit.py:607-609);
+    # the workers inherit this environment when they start
+    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in saved:
+        os.environ[k] = "1"
+    t0 = time.time()
+    with mp.get_context("spawn").Pool(cores) as pool:
+        res = pool.map(_cpu_chain, [(L, Lt, ng, ne, seed + i) for i in range(cores)])
+    wall = time.time() - t0
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+    dts = [r[0] for r in res]
+    rates = [1.0 / d for d in dts]
+    return {"value": float(np.mean(rates)), "unit": "chain-sweeps/s", "cores": cores, "kind": "port",
+            "aggregate": float(np.sum(rates)), "cpu_model": _cpu_model(),
+            "sweep_s": {"min": float(min(dts)), "max": float(max(dts))},
+            "process_start_s": {"min": min(r[1] for r in res) - t0, "max": max(r[1] for r in res) - t0},
+            "init_s": {"min": min(r[2] - r[1] for r in res), "max": max(r[2] - r[1] for r in res)},
+            "sample": f"oracle (C, 1 thread per process), {cores} chains (seeds {seed}..{seed + cores - 1}) on "
+                      f"{cores} cores at once, one full backward sweep each with the Lt/2 measurement "
+                      f"(init not timed; {wall:.1f} s wall); value = mean per-core rate"}
+
+
+def default_cpu_cores():
+    """Host cores of this process's share: the affinity set, capped by the
+    thread count the environment grants (OMP_NUM_THREADS) and by 16."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        n = min(n, int(omp))
+    return max(1, min(n, 16))
+
+
+def mixed_batch_leg(nchains, threads, helper_bonds, sweeps):
+    """The mixed batch: nchains independent chains (32 each at L = 8, 10, 12,
+    14; Lt cycling through 400..1000; seeds 1000 + i) advanced by
+    pqmc_sweep_many, `threads` at a time.  Reports aggregate chain-sweeps/s
+    and Lt=1000-equivalent sweeps/s (sum of Lt / 1000 per second)."""
+    from pqmc_allinone_amd.host import Chain, Ran2, sweep_many
+    from pqmc_allinone_amd.model import ModelParams, build_model, mixed_chain_specs, synthetic_inputs
+    specs = mixed_chain_specs(nchains)
+    models, chains, rngs = {}, [], []
     t0 = time.perf_counter()
-    orc2.sweep(False, u, ntc=ngroups * ng)
-    dt = time.perf_counter() - t0
-    frac = ngroups * ng / Lt
-    return {"value": frac / dt, "unit": "chain-sweeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C, 1 thread) over the first {ngroups * ng} of {Lt} slices of a backward sweep "
-                      f"({dt:.1f} s), scaled by slices"}
+    for L, Lt, seed in specs:
+        if (L, Lt) not in models:
+            models[(L, Lt)] = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=10, ne=L * L))
+        m = models[(L, Lt)]
+        x, sl = synthetic_inputs(m, seed)
+        ch = Chain(m, helper_bonds=helper_bonds)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        ch.init_g(1)
+        chains.append(ch)
+        rngs.append(Ran2(seed))
+    t_init = time.perf_counter() - t0
+    fw = True
+    dt = 0.0
+    for _ in range(sweeps):
+        fw = not fw
+        draws = [r.draws(2 * c.Nb * c.model.p.Lt) for r, c in zip(rngs, chains)]
+        t0 = time.perf_counter()
+        sweep_many(chains, [fw] * nchains, draws, measure=True, max_concurrent=threads)
+        dt += time.perf_counter() - t0
+    drift = max(c.counters()["err_fast_max"] for c in chains)
+    for c in chains:
+        c.close()
+    lt_sum = sum(Lt for _, Lt, _ in specs)
+    return {"chains": nchains, "threads": threads, "helper_bonds": helper_bonds, "sweeps": sweeps,
+            "value": nchains * sweeps / dt, "unit": "chain-sweeps/s",
+            "lt1000_sweeps_per_s": lt_sum / 1000.0 * sweeps / dt, "s_per_sweep": dt / sweeps,
+            "init_s": t_init, "err_fast_max": drift, "dtype": "c128 (fp64)",
+            "workload": "32 chains each at L = 8, 10, 12, 14 (Ns = 2 L^2, ne = L^2), Lt cycling 400/600/800/1000, "
+                        "ngroup 10, seeds 1000 + i"}
 
 
 def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_plain):
@@ -237,7 +333,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--block-bonds", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="host cores (one oracle chain each) of the CPU baseline; 0 = this process's share, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--batch", type=int, default=9,
@@ -250,13 +347,18 @@ def main():
                     help="field-gradient passes on one chain (0: skip the gradient leg)")
     ap.add_argument("--grad-chains", type=int, default=16, help="chains of the concurrent gradient pass")
     ap.add_argument("--hmc-trajectories", type=int, default=1, help="HMC trajectories (0: skip the HMC leg)")
+    ap.add_argument("--mixed-chains", type=int, default=128,
+                    help="chains of the mixed batch leg (32 per lattice of L = 8..14; 0: no such leg)")
+    ap.add_argument("--mixed-threads", type=int, default=16, help="chains of the mixed batch swept at once")
+    ap.add_argument("--mixed-sweeps", type=int, default=2, help="timed sweeps of the mixed batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
     args = ap.parse_args()
     # one hardware queue per chain of the batched and gradient legs (read when
     # the HIP runtime starts, so before the first device call)
-    nq = max(args.batch if args.batch > 1 else 0, args.grad_chains if args.grad_passes > 0 else 0)
+    nq = max(args.batch if args.batch > 1 else 0, args.grad_chains if args.grad_passes > 0 else 0,
+             args.mixed_threads if args.mixed_chains > 0 else 0)
     if nq > 1:
         q = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
         if q < nq + 2:
@@ -275,10 +377,12 @@ def main():
     L, Lt, ng, ne, seed = CONFIGS[args.config]
     m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne, lam=0.64, omega=1.0))
     x, sl = synthetic_inputs(m, seed)
+    log(f"plain chain {L}x{L} Lt={Lt}: init_g")
     ch = Chain(m, block_bonds=args.block_bonds)
     ch.set_fields(x)
     ch.set_trial(sl)
     ch.init_g(1)
+    log(f"plain chain: {args.warmup} warm-up and {args.steps} timed sweeps")
     rng = Ran2(seed)
     nd = 2 * m.Nb * Lt
     forward = True
@@ -326,23 +430,36 @@ def main():
         roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None,
                 "flops_per_launch": fl}
+    log(f"plain chain: {sps:.3f} chain-sweeps/s")
     batched = None
     if args.batch > 1:
+        log(f"batched leg: {args.batch} chains")
         batched = batched_leg(m, x, sl, seed, args.batch, args.warmup, args.steps, args.block_bonds,
                               args.batch_helper_bonds, ch.g_h)
         batched["vs_plain"] = batched["value"] / sps
     ft = None
     if args.ft_steps > 0:
+        log("finite-temperature leg")
         ft = ft_leg(m, x, seed, args.warmup, args.ft_steps)
     grad = None
     if args.grad_passes > 0:
+        log("gradient leg")
         grad = grad_leg(m, x, sl, args.grad_passes, args.grad_chains)
     hmc = None
     if args.hmc_trajectories > 0:
+        log("HMC leg")
         hmc = hmc_leg(m, x, sl, args.hmc_trajectories)
+    mixed = None
+    if args.mixed_chains > 0:
+        log(f"mixed batch leg: {args.mixed_chains} chains")
+        mixed = mixed_batch_leg(args.mixed_chains, args.mixed_threads, 4, args.mixed_sweeps)
+        log(f"mixed batch: {mixed['value']:.2f} chain-sweeps/s")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(L, Lt, ng, ne, seed, args.cpu_seconds)
+        nc = args.cpu_cores or default_cpu_cores()
+        log(f"CPU baseline: {nc} oracle chains, one full sweep each")
+        cpu = cpu_baseline(L, Lt, ng, ne, seed, nc)
+        log(f"CPU baseline: {cpu['value']:.4f} chain-sweeps/s per core")
     out = {
         "metric": "chain-sweeps/sec at 14x14 Lt=1000 (plain FP64 single chain)",
         "value": sps,
@@ -364,6 +481,7 @@ def main():
         "acceptance": n_acc / max(n_prop, 1),
         "kernels": kern,
         "batched": batched,
+        "mixed_batch": mixed,
         "finite_temperature": ft,
         "gradient": grad,
         "hmc": hmc,
