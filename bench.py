@@ -174,7 +174,50 @@ def mixed_batch_leg(nchains, threads, helper_bonds, sweeps):
                         "ngroup 10, seeds 1000 + i"}
 
 
-def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_plain):
+def precision_leg(L, Lt, seed, modes=("fp32x3", "bf16x3")):
+    """The mixed-precision B-string products against FP64 on one chain of
+    L x L, Lt slices, fields and draws from `seed`: max|G - G_FP64| after
+    init_g(1) and after one backward sweep with the same draws, how many of
+    the sweep's accept decisions agree, and the largest relative difference
+    of the Lt/2 measurement (entries above 1e-6 of the largest)."""
+    from pqmc_allinone_amd.host import Chain, Ran2
+    from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=10, ne=L * L))
+    x, sl = synthetic_inputs(m, seed)
+    u = Ran2(seed).draws(2 * m.Nb * Lt)
+
+    def run(mode):
+        ch = Chain(m)
+        ch.set_precision(mode)
+        ch.set_fields(x)
+        ch.set_trial(sl)
+        t0 = time.perf_counter()
+        ch.init_g(1)
+        g_init = ch.g_h
+        t1 = time.perf_counter()
+        meas = ch.sweep(False, u, measure=True)
+        ch.synchronize()
+        t2 = time.perf_counter()
+        out = (g_init, ch.g_h, ch.accepts(), meas, t1 - t0, t2 - t1, ch.drift()[0])
+        ch.close()
+        return out
+
+    ref = run("fp64")
+    res = {"lattice": f"{L}x{L}", "Lt": Lt, "seed": seed, "fp64_sweep_s": ref[5]}
+    big = np.abs(ref[3]) > 1e-6 * np.abs(ref[3]).max()
+    for mode in modes:
+        r = run(mode)
+        same = r[2] == ref[2]
+        res[mode] = {"g_err_init": float(np.abs(r[0] - ref[0]).max()),
+                     "g_err_after_sweep": float(np.abs(r[1] - ref[1]).max()),
+                     "accepts_agree": float(same.mean()),
+                     "first_accept_difference": None if same.all() else int(np.argmin(same)),
+                     "measurement_max_rel_diff": float((np.abs(r[3] - ref[3])[big] / np.abs(ref[3])[big]).max()),
+                     "drift_g": r[6], "sweep_s": r[5]}
+    return res
+
+
+def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_plain, precision="fp32x3"):
     """K chains of the workload, one batch sweep per step; chain 0 replays the
     plain chain (same fields, trial and draws), chains i > 0 take fields and
     draws from seed + i."""
@@ -184,6 +227,7 @@ def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_
     for i in range(K):
         xi = x0 if i == 0 else synthetic_inputs(m, seed + i)[0]
         ch = Chain(m, block_bonds=block_bonds, helper_bonds=helper_bonds)
+        ch.set_precision(precision)
         ch.set_fields(xi)
         ch.set_trial(sl)
         ch.init_g(1)
@@ -207,7 +251,10 @@ def batched_leg(m, x0, sl, seed, K, warmup, steps, block_bonds, helper_bonds, g_
     for ch in chains:
         ch.close()
     return {"chains": K, "helper_bonds": helper_bonds, "value": K * steps / dt, "unit": "chain-sweeps/s",
-            "ms_per_step": 1e3 * dt / steps, "dtype": "c128 (fp64)", "max_abs_g_err_vs_plain": g_err}
+            "ms_per_step": 1e3 * dt / steps, "precision": precision,
+            "dtype": "c128 (fp64); B-string products between checkpoints on FP32 MFMA with three-term split "
+                     "operands" if precision == "fp32x3" else "c128 (fp64)",
+            "max_abs_g_err_vs_plain": g_err}
 
 
 def ft_leg(m, x, seed, warmup, steps):
@@ -339,6 +386,10 @@ def main():
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--batch", type=int, default=9,
                     help="chains of the batched leg (0 or 1: no batched leg)")
+    ap.add_argument("--batch-precision", default="fp32x3", choices=("fp64", "fp32x3", "bf16x3"),
+                    help="B-string products of the batched leg's chains")
+    ap.add_argument("--precision-lattices", default="14,20",
+                    help="lattices of the precision leg (Lt=1000, seed 11; empty: no such leg)")
     ap.add_argument("--batch-helper-bonds", type=int, default=4,
                     help="bonds per column-owning workgroup in the batched leg's family passes (1, 2 or 4)")
     ap.add_argument("--ft-steps", type=int, default=2,
@@ -435,7 +486,7 @@ def main():
     if args.batch > 1:
         log(f"batched leg: {args.batch} chains")
         batched = batched_leg(m, x, sl, seed, args.batch, args.warmup, args.steps, args.block_bonds,
-                              args.batch_helper_bonds, ch.g_h)
+                              args.batch_helper_bonds, ch.g_h, args.batch_precision)
         batched["vs_plain"] = batched["value"] / sps
     ft = None
     if args.ft_steps > 0:
@@ -449,6 +500,12 @@ def main():
     if args.hmc_trajectories > 0:
         log("HMC leg")
         hmc = hmc_leg(m, x, sl, args.hmc_trajectories)
+    prec = None
+    if args.precision_lattices:
+        prec = []
+        for Lp in (int(v) for v in args.precision_lattices.split(",")):
+            log(f"precision leg: {Lp}x{Lp} Lt=1000 seed 11")
+            prec.append(precision_leg(Lp, 1000, 11))
     mixed = None
     if args.mixed_chains > 0:
         log(f"mixed batch leg: {args.mixed_chains} chains")
@@ -461,7 +518,8 @@ def main():
         cpu = cpu_baseline(L, Lt, ng, ne, seed, nc)
         log(f"CPU baseline: {cpu['value']:.4f} chain-sweeps/s per core")
     out = {
-        "metric": "chain-sweeps/sec at 14x14 Lt=1000 (plain FP64 single chain)",
+        "metric": "chain-sweeps/sec at 14x14 Lt=1000: plain FP64 vs batched mixed-prec @ G err<=1e-8",
+        "value_is": "plain FP64, one chain (the batched mixed-precision figure is batched.value)",
         "value": sps,
         "unit": "chain-sweeps/s",
         "n_gpus": 1,
@@ -482,6 +540,7 @@ def main():
         "kernels": kern,
         "batched": batched,
         "mixed_batch": mixed,
+        "precision": prec,
         "finite_temperature": ft,
         "gradient": grad,
         "hmc": hmc,

@@ -83,6 +83,16 @@ int pqmc_get_fields(pqmc_handle *h, double *x);
  * set_nelec_occupied (src/Main_PQMC.f90:301-302). */
 int pqmc_set_trial(pqmc_handle *h, const double *slater);
 
+/* Precision of the B-string products between QR checkpoints, the one-way
+ * left_evolve / right_evolve loops of init_g_T0 and get_g_scratch_T0
+ * (src/mod_evolution.f90:88-102, 137-162) and of cal_new_weight_T0
+ * (:359-408): 0 = FP64 (default); 1 = the slices' product P formed in FP64
+ * and P X taken on FP32 MFMA with three-term split operands; 2 = the same on
+ * BF16 MFMA with BF16x3 operands.  QR, D, the (L R)^-1 solve, the updates and
+ * every determinant ratio stay FP64.  Projector chains only.  No reference
+ * counterpart (the reference computes in FP64 throughout). */
+int pqmc_set_precision(pqmc_handle *h, int mode);
+
 /* init_MC: init_expKV + set_pf_expKV_mat + init_g(start_time)
  * (src/Main_PQMC.f90:185-203, src/mod_evolution.f90:46-111). */
 int pqmc_init_g(pqmc_handle *h, int start_time);

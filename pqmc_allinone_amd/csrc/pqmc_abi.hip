@@ -64,6 +64,8 @@ hipError_t pqmc_ft_dlog(const zd *Dr, const zd *Dl, const zd *Lr, const zd *Ll, 
 hipError_t pqmc_ft_qlog(const zd *tau, const double *vn2, int n, int conj, zd *out, hipStream_t s);
 hipError_t pqmc_ft_ones(zd *D, int n, hipStream_t s);
 hipError_t pqmc_ft_finish(zd *G, int n, hipStream_t s);
+hipError_t pqmc_mx_product(int mode, int M, int N, int K, const zd *P, int lda, const zd *Q, int ldb, zd *C, int ldc,
+                           double *scales, hipStream_t s);
 }
 
 static thread_local std::string g_err;
@@ -151,6 +153,12 @@ struct pqmc_handle {
   zd *d_fw = nullptr;    // five Ns x Ns work matrices
   zd *d_flog = nullptr;  // [4] log terms of ln det(1 + B)
   int *d_fperm = nullptr;  // [2 Ns] pre-pivoting order and scale exponents of a string step
+  // precision of the B-string products between checkpoints: 0 FP64, 1 FP32
+  // three-term split, 2 BF16x3 (k_mixed.hip); the dense group product P
+  // (Ns x Ns), the product's output (Ns x ne) and the split scales
+  int prec = 0;
+  zd *d_mxP = nullptr, *d_mxX = nullptr;
+  double *d_mxs = nullptr;
   qr_ws qf;              // QR workspace for Ns x Ns
   // timing
   bool timing[3] = {false, false, false};
@@ -364,7 +372,8 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_hist, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
-                  h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad};
+                  h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad,
+                  h->d_mxP, h->d_mxX, h->d_mxs};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -510,6 +519,37 @@ static int propagate(pqmc_handle *h, zd *X, int t_first, int nslices, int backwa
   a.backward = backward;
   a.fam_only = -1;
   HIPCHK(pqmc_launch_propagate(&a, h->st));
+  return PQMC_OK;
+}
+
+// One B-string step X (Ns x ne) <- P X over nslices slices, P the product of
+// their family layers (the one-way left_evolve / right_evolve loops of
+// init_g_T0 and get_g_scratch_T0): applied layer by layer in FP64, or, in a
+// mixed-precision mode, P formed in FP64 (the layers applied to the
+// identity) and the product P X taken on FP32 or BF16 MFMA with split
+// operands (k_mixed.hip).
+static int string_step(pqmc_handle *h, zd *X, int t_first, int nslices, int backward, const ekv_t *ekv = nullptr) {
+  if (h->prec == 0) return propagate(h, X, t_first, nslices, backward, ekv);
+  const int Ns = h->Ns, ne = h->ne;
+  HIPCHK(pqmc_eye(h->d_mxP, Ns, Ns, Ns, h->st));
+  int rc = propagate(h, h->d_mxP, t_first, nslices, backward, ekv, Ns);
+  if (rc) return rc;
+  HIPCHK(pqmc_mx_product(h->prec, Ns, ne, Ns, h->d_mxP, Ns, X, Ns, h->d_mxX, Ns, h->d_mxs, h->st));
+  HIPCHK(hipMemcpyAsync(X, h->d_mxX, sizeof(zd) * Ns * ne, hipMemcpyDeviceToDevice, h->st));
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_precision(pqmc_handle *h, int mode) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (mode < 0 || mode > 2) return fail(PQMC_EINVAL, "precision mode must be 0 (FP64), 1 (FP32 split) or 2 (BF16x3)");
+  if (mode && h->ft) return fail(PQMC_EINVAL, "mixed-precision B-string products are for projector chains");
+  if (mode && !h->d_mxP) {
+    const size_t Ns = h->Ns;
+    HIPCHK(dalloc(&h->d_mxP, Ns * Ns));
+    HIPCHK(dalloc(&h->d_mxX, Ns * h->ne));
+    HIPCHK(dalloc(&h->d_mxs, Ns + h->ne));
+  }
+  h->prec = mode;
   return PQMC_OK;
 }
 
@@ -726,7 +766,7 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   int p = stb;
   while (p <= h->Lt) {
     int pend = std::min(h->Lt, ((p - 1) / ng + 1) * ng);
-    int rc = propagate(h, h->d_W, p, pend - p + 1, 0);
+    int rc = string_step(h, h->d_W, p, pend - p + 1, 0);
     if (rc) return rc;
     rmat = h->d_W;
     if (pend % ng == 0) {
@@ -886,7 +926,7 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     const int block = time / ng + 1;
     if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
     HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block - 1), qb, hipMemcpyDeviceToDevice, h->st));
-    rc = propagate(h, h->d_W, time - ng, ng, 0);
+    rc = string_step(h, h->d_W, time - ng, ng, 0);
     if (rc) return rc;
     rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
     if (rc) return rc;
@@ -898,7 +938,7 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     if (ng == 1) block -= 1;
     if (block - 1 < 1 || block + 1 > h->nblock) return fail(PQMC_EINVAL, "stabilization block out of range");
     HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block + 1), qb, hipMemcpyDeviceToDevice, h->st));
-    rc = propagate(h, h->d_W, time + ng - 1, ng, 1);
+    rc = string_step(h, h->d_W, time + ng - 1, ng, 1);
     if (rc) return rc;
     rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 1);
     if (rc) return rc;
@@ -1294,7 +1334,7 @@ extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
   int nlog = 2;
   for (int p = Lt; p >= 1;) {
     const int n = std::min(ng, p);
-    rc = propagate(h, W, p, n, 1, h->d_gekv);
+    rc = string_step(h, W, p, n, 1, h->d_gekv);
     if (rc) return rc;
     p -= n;
     if (n < ng) break;  // a partial last group has no LQ

@@ -321,6 +321,15 @@ class Chain:
         _lib.check(self._L.pqmc_get_drift(self.h, _lib.ptr(o)))
         return float(o[0]), float(o[1])
 
+    PRECISIONS = {"fp64": 0, "fp32x3": 1, "bf16x3": 2}
+
+    def set_precision(self, mode: str):
+        """B-string products between QR checkpoints in "fp64" (default),
+        "fp32x3" (FP32 MFMA, three-term split operands) or "bf16x3" (BF16
+        MFMA); everything else stays FP64.  Takes effect from the next
+        init_g / get_g_scratch / sweep."""
+        _lib.check(self._L.pqmc_set_precision(self.h, self.PRECISIONS[mode]))
+
     def drift_history(self) -> np.ndarray:
         """(n, 2): G and Re ln_cw drift of each stabilization of the latest
         sweep, gradient pass or get_g_scratch, in order."""
