@@ -206,6 +206,16 @@ int pqmc_global_weight(pqmc_handle *h, const double *x, double *re_im);
  * gradient pass, with no init_g or forward sweep since. */
 int pqmc_gradient(pqmc_handle *h, double *grad);
 
+/* pqmc_gradient without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the device accumulator is the library's fixed 16 MiB
+ * array, one per device, instead of a buffer allocated on the chain's first
+ * gradient pass.  Same arguments, result and call-order rules as
+ * pqmc_gradient; PQMC_EINVAL when Nb * Lt exceeds 1,048,576 fields.  Calls
+ * on one device share the array: calls that overlap on a device race.  No
+ * counterpart in the reference. */
+int pqmc_gradient_noalloc(pqmc_handle *h, double *grad);
+
 /* The ran2 generator of src/mod_ranstate.f90:171-191 seeded as
  * init_rng_by_seed (:267-283); host-side, for drivers without their own. */
 typedef struct pqmc_ran2 { int32_t i, j, k, m, n; } pqmc_ran2;

@@ -67,6 +67,7 @@ SIGNATURES = {
     "pqmc_get_lncw": (_I, [_P, _P]),
     "pqmc_global_weight": (_I, [_P, _P, _P]),
     "pqmc_gradient": (_I, [_P, _P]),
+    "pqmc_gradient_noalloc": (_I, [_P, _P]),
     "pqmc_get_accepts": (_I, [_P, _P, C.c_int64]),
     "pqmc_get_counters": (_I, [_P, _P, _P]),
     "pqmc_get_drift": (_I, [_P, _P]),

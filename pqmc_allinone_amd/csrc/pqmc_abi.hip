@@ -1057,22 +1057,17 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
 // without updates; each family's two-way wrap E^-1 G E first adds the
 // family's gradient from the G(S, S) blocks it holds (k_flush_wrap's diagonal
 // tiles), and G is recomputed every ngroup slices as in a backward sweep.
-extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
-  if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
-  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
-  if (h->g_at == 2)
-    return fail(PQMC_EINVAL, "G is at the bottom (after a backward sweep or a gradient pass): the gradient pass "
-                             "starts where a backward sweep starts, after init_g or a forward sweep");
+// acc: device accumulator of Nb*Lt complex values, cleared here.
+static int gradient_pass(pqmc_handle *h, zd *acc, double *grad) {
   const int Lt = h->Lt;
   const size_t n = (size_t)h->Nb * Lt;
-  if (!h->d_grad) HIPCHK(dalloc(&h->d_grad, n));
-  HIPCHK(hipMemsetAsync(h->d_grad, 0, sizeof(zd) * n, h->st));
+  HIPCHK(hipMemsetAsync(acc, 0, sizeof(zd) * n, h->st));
   int rc = PQMC_OK;
   h->nhist = 0;
   for (int tc = 1; tc <= Lt; ++tc) {
     const int time = Lt - tc + 1;
     for (int f = 3; f >= 0; --f) {
-      rc = wrap_g(h, 2, f, time, h->d_grad);
+      rc = wrap_g(h, 2, f, time, acc);
       if (rc) return rc;
     }
     if (tc % h->ngroup == 0) {
@@ -1084,10 +1079,71 @@ extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
     rc = read_err(h, nullptr);
     if (rc) return rc;
   }
-  HIPCHK(hipMemcpyAsync(grad, h->d_grad, sizeof(zd) * n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(grad, acc, sizeof(zd) * n, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   h->g_at = 2;
   return PQMC_OK;
+}
+
+static int gradient_check(pqmc_handle *h, double *grad) {
+  if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
+  if (!h->g_ready) return fail(PQMC_EINVAL, "init_g has not run");
+  if (h->g_at == 2)
+    return fail(PQMC_EINVAL, "G is at the bottom (after a backward sweep or a gradient pass): the gradient pass "
+                             "starts where a backward sweep starts, after init_g or a forward sweep");
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_gradient(pqmc_handle *h, double *grad) {
+  int rc = gradient_check(h, grad);
+  if (rc) return rc;
+  if (!h->d_grad) HIPCHK(dalloc(&h->d_grad, (size_t)h->Nb * h->Lt));
+  return gradient_pass(h, h->d_grad, grad);
+}
+
+// The same pass for callers that may not allocate device memory: the
+// accumulator is the library's fixed device array `scratch` (16 MiB, one per
+// device, held until the process ends) instead of a per-chain buffer.  Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the device (taken once per device, with that device current); the array's
+// name is a host-side stand-in.  Calls on one device share the array, so
+// calls that overlap on a device race.
+#define PQMC_NOALLOC_GRAD_MAX (1 << 20)  // complex entries: Nb * Lt up to 1,048,576
+static __device__ __align__(256) zd scratch[PQMC_NOALLOC_GRAD_MAX];
+#define PQMC_MAX_DEVICES 64
+static std::mutex g_noalloc_mu;
+static zd *g_noalloc_ptr[PQMC_MAX_DEVICES];
+
+static int noalloc_scratch(int dev, zd **out) {
+  if (dev < 0 || dev >= PQMC_MAX_DEVICES) return fail(PQMC_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(PQMC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = static_cast<zd *>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_gradient_noalloc(pqmc_handle *h, double *grad) {
+  if (!h || !grad) return fail(PQMC_EINVAL, "null argument");
+  if ((size_t)h->Nb * h->Lt > (size_t)PQMC_NOALLOC_GRAD_MAX)
+    return fail(PQMC_EINVAL, "Nb * Lt = " + std::to_string((size_t)h->Nb * h->Lt) +
+                                 " fields exceed the allocation-free gradient's fixed array of " +
+                                 std::to_string(PQMC_NOALLOC_GRAD_MAX));
+  int rc = gradient_check(h, grad);
+  if (rc) return rc;
+  zd *acc = nullptr;
+  rc = noalloc_scratch(h->dev, &acc);
+  if (rc) return rc;
+  return gradient_pass(h, acc, grad);
 }
 
 // Compute units one chain's family passes keep resident: every workgroup of

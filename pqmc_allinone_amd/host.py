@@ -281,13 +281,16 @@ class Chain:
         _lib.check(self._L.pqmc_global_weight(self.h, _lib.ptr(x), _lib.ptr(v)))
         return complex(v[0], v[1])
 
-    def gradient(self) -> np.ndarray:
+    def gradient(self, noalloc: bool = False) -> np.ndarray:
         """d ln_cw / dx for every field, complex, shape (Lt, Nb): a backward
         pass with no updates (pqmc_gradient).  Call where a backward sweep
         starts (after init_g(1) or a forward sweep); the chain ends where a
-        backward sweep that accepts nothing leaves it."""
+        backward sweep that accepts nothing leaves it.  noalloc: the
+        library's fixed device array as the accumulator
+        (pqmc_gradient_noalloc), no device allocation."""
         g = np.empty((self.model.p.Lt, self.Nb), np.complex128)
-        _lib.check(self._L.pqmc_gradient(self.h, _lib.ptr(g)))
+        fn = self._L.pqmc_gradient_noalloc if noalloc else self._L.pqmc_gradient
+        _lib.check(fn(self.h, _lib.ptr(g)))
         return g
 
     def q_string(self, block: int) -> np.ndarray:
