@@ -662,43 +662,35 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         // hand accept m to wave 1
         if (lane == 0) __hip_atomic_store(&L.npub, m + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         PROF_ACC(6);
-        // W += u_m v_m on the later rows and columns; lane (rr, cl) of an 8 x 8
-        // grid owns rows o + rr + 8i and columns o + cl + 8i2.  Operands are
-        // read into registers before any write (all addresses clamped in range).
+        // W += u_m v_m on the later rows and columns, on FP64 MFMA: per 16 x 16
+        // tile of W that meets them, D += U V with U the u_m rows (zero above
+        // row o) and V the v_m columns (zero left of column o), complex as four
+        // real products of K = 4.  Lane l supplies U[l % 16][l / 16] and
+        // V[l / 16][l % 16] and holds rows l / 16 + 4 i of column l % 16.
         if (Ln > 0) {
-          zd *__restrict__ Wr = W;
-          const zd *__restrict__ Ur = L.sU[m];
-          const zd *__restrict__ Vr = L.sV[m];
-          const int rr = lane >> 3, cl = lane & 7;
-          const int ni = (Ln + 7) >> 3;
-          zd vv[4][4];
+          const int li = lane & 15, lk = lane >> 4;
+          const int t0 = o >> 4, t1 = (nsk - 1) >> 4;
+          for (int rt = t0; rt <= t1; ++rt) {
+            const int ra = 16 * rt + li;
+            const zd ua = zsel(ra >= o && ra < nsk, L.sU[m][ra * 4 + lk]);
+            for (int ct = t0; ct <= t1; ++ct) {
+              const int cb = 16 * ct + li;
+              const zd vb = zsel(cb >= o && cb < nsk, L.sV[m][lk * PS_BS + cb]);
+              double4_t dre, dim;
 #pragma unroll
-          for (int i2 = 0; i2 < 4; ++i2) {
-            const int c = o + cl + 8 * i2;
-            const bool cv = i2 < ni && c < nsk;
-            const int cc = min(c, PS_BS - 1);
+              for (int i = 0; i < 4; ++i) {
+                const zd t = W[(16 * rt + lk + 4 * i) * PS_BS + cb];
+                dre[i] = t.x;
+                dim[i] = t.y;
+              }
+              dre = mfma_f64(ua.x, vb.x, dre);
+              dim = mfma_f64(ua.x, vb.y, dim);
+              dre = mfma_f64(-ua.y, vb.y, dre);
+              dim = mfma_f64(ua.y, vb.x, dim);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              const zd t = Vr[a * PS_BS + cc];
-              vv[i2][a] = cv ? t : zmk(0.0, 0.0);
-            }
-          }
-          for (int i = 0; i < ni; ++i) {
-            const int r = min(o + rr + 8 * i, PS_BS - 1);
-            const bool rv = o + rr + 8 * i < nsk;
-            zd uu[4], w[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) uu[a] = Ur[r * 4 + a];
-#pragma unroll
-            for (int i2 = 0; i2 < 4; ++i2) w[i2] = Wr[r * PS_BS + min(o + cl + 8 * i2, PS_BS - 1)];
-#pragma unroll
-            for (int i2 = 0; i2 < 4; ++i2) {
-              if (i2 < ni) {
-                const int c = o + cl + 8 * i2;
-                zd s = w[i2];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) s = zadd(s, zmul(uu[a], vv[i2][a]));
-                if (rv && c < nsk) Wr[r * PS_BS + c] = s;
+              for (int i = 0; i < 4; ++i) {
+                const int row = 16 * rt + lk + 4 * i;
+                if (row >= o && row < nsk && cb >= o && cb < nsk) W[row * PS_BS + cb] = zmk(dre[i], dim[i]);
               }
             }
           }
