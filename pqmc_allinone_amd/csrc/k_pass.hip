@@ -840,7 +840,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
 // --------------------------------------------------------------------------
 struct ps_helper_hdr {
   zd sAT[PS_BS * PS_BS];       // A and T^-1 of the block (packed as in the record, stride PS_BS)
-  zd sB[PS_BS * PS_BS];        // B = A T^-1 (row-major, stride PS_BS)
+  zd sB[PS_BS * PS_BS];        // v = T^-1 w per own column (stride PS_BS)
   int li[PS_NB];
   int a, ok, sink;
 };
@@ -1072,43 +1072,47 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
       for (int e = tid; e < n4 * n4; e += PS_THREADS) Hh.sAT[(e / n4) * PS_BS + e % n4] = R->AT[e];
       __syncthreads();
       PROF_ACC(2);
-      // w[q][g] = e_q(S_g) - G_k[S_g, q] and B = A T^-1:
-      //   B[g][h] = A[g][h] (h/4 >= g/4) + sum_{j/4 > h/4, j/4 >= g/4} A[g][j] T^-1[j][h]
+      // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q] = A (T^-1 w[q]) as
+      // two block-triangular products (no n4 x n4 product B = A T^-1):
+      //   v[q][g] = w[q][g] + sum_{h/4 < g/4} T^-1[g][h] w[q][h]
+      //   y[q][g] = sum_{h/4 >= g/4} A[g][h] v[q][h]
       const int *Sk = sS + 4 * A.nb * k;
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
         const int q = e / n4, g = e % n4;
         const int st = Sk[4 * Hh.li[g >> 2] + (g & 3)];
         sw[q * PS_BS + g] = zsub(zmk(sS[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
       }
-      for (int e = tid; e < n4 * n4; e += PS_THREADS) {
-        const int g = e / n4, h = e % n4;
-        const int gb = g >> 2, hb = h >> 2;
-        zd b0 = hb >= gb ? Hh.sAT[g * PS_BS + h] : zmk(0.0, 0.0), b1 = zmk(0.0, 0.0);
-        for (int j = 4 * max(gb, hb + 1); j < n4; j += 4) {
-          zd av[4], tv[4];
+      __syncthreads();
+      zd *sv = Hh.sB;  // v[q][g] at sv[q * PS_BS + g]
+      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
+        const int q = e / n4, g = e % n4;
+        const int hend = 4 * (g >> 2);
+        zd v0 = sw[q * PS_BS + g], v1 = zmk(0.0, 0.0);
+        for (int hh = 0; hh < hend; hh += 4) {
+          zd tv[4], wv[4];
 #pragma unroll
           for (int d = 0; d < 4; ++d) {
-            av[d] = Hh.sAT[g * PS_BS + j + d];
-            tv[d] = Hh.sAT[(j + d) * PS_BS + h];
+            tv[d] = Hh.sAT[g * PS_BS + hh + d];
+            wv[d] = sw[q * PS_BS + hh + d];
           }
-          b0 = zfma(zfma(b0, av[0], tv[0]), av[1], tv[1]);
-          b1 = zfma(zfma(b1, av[2], tv[2]), av[3], tv[3]);
+          v0 = zfma(zfma(v0, tv[0], wv[0]), tv[1], wv[1]);
+          v1 = zfma(zfma(v1, tv[2], wv[2]), tv[3], wv[3]);
         }
-        Hh.sB[g * PS_BS + h] = zadd(b0, b1);
+        sv[q * PS_BS + g] = zadd(v0, v1);
       }
       __syncthreads();
       for (int e = tid; e < ncol * n4; e += PS_THREADS) {
         const int q = e / n4, g = e % n4;
         zd y0 = zmk(0.0, 0.0), y1 = y0;
-        for (int hh = 0; hh < n4; hh += 4) {
-          zd bv[4], wv[4];
+        for (int hh = 4 * (g >> 2); hh < n4; hh += 4) {
+          zd av[4], vv[4];
 #pragma unroll
           for (int d = 0; d < 4; ++d) {
-            bv[d] = Hh.sB[g * PS_BS + hh + d];
-            wv[d] = sw[q * PS_BS + hh + d];
+            av[d] = Hh.sAT[g * PS_BS + hh + d];
+            vv[d] = sv[q * PS_BS + hh + d];
           }
-          y0 = zfma(zfma(y0, bv[0], wv[0]), bv[1], wv[1]);
-          y1 = zfma(zfma(y1, bv[2], wv[2]), bv[3], wv[3]);
+          y0 = zfma(zfma(y0, av[0], vv[0]), av[1], vv[1]);
+          y1 = zfma(zfma(y1, av[2], vv[2]), av[3], vv[3]);
         }
         sy[q * PS_BS + g] = zadd(y0, y1);
       }
