@@ -359,6 +359,35 @@ def hmc_leg(m, x, sl, ntraj):
             "accepted": hmc.accepted, "leapfrog": "10 x 0.05, Fourier-accelerated mass"}
 
 
+def kernel_classes(reps):
+    """The path's kernel classes other than the family pass, on the 14x14,
+    Lt=1000 chain, for counter collection: init_g(1), then `reps` each of
+    get_g_scratch (B-string propagation, blocked Householder QR, G from
+    scratch), take_measurement, the standalone two-way wrap, the field
+    gradient's wrap (a gradient pass over the first 10 slices), and the
+    mixed-precision string products (FP32 split and BF16x3)."""
+    from pqmc_allinone_amd.host import Chain
+    from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+    L, Lt, ng, ne, seed = CONFIGS["c2"]
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne))
+    x, sl = synthetic_inputs(m, seed)
+    ch = Chain(m)
+    ch.set_fields(x)
+    ch.set_trial(sl)
+    ch.init_g(1)
+    for r in range(reps):
+        ch.get_g_scratch(Lt - 10 * (r + 1) + 1, False)
+        ch.take_measurement(Lt // 2)
+        ch.right_evolve(Lt - r, 4)
+    for mode in ("fp32x3", "bf16x3"):
+        ch.set_precision(mode)
+        for r in range(reps):
+            ch.get_g_scratch(Lt - 10 * (r + 1) + 1, False)
+    ch.set_precision("fp64")
+    ch.synchronize()
+    print(json.dumps({"kernel_classes": reps}))
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -402,6 +431,8 @@ def main():
                     help="chains of the mixed batch leg (32 per lattice of L = 8..14; 0: no such leg)")
     ap.add_argument("--mixed-threads", type=int, default=16, help="chains of the mixed batch swept at once")
     ap.add_argument("--mixed-sweeps", type=int, default=2, help="timed sweeps of the mixed batch")
+    ap.add_argument("--kernel-classes", type=int, default=0, metavar="REPS",
+                    help="only run REPS of each kernel class other than the family pass (counter collection)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
@@ -416,6 +447,9 @@ def main():
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, nq + 3))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.kernel_classes:
+        kernel_classes(args.kernel_classes)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs with no "
