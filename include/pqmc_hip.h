@@ -93,6 +93,14 @@ int pqmc_set_trial(pqmc_handle *h, const double *slater);
  * counterpart (the reference computes in FP64 throughout). */
 int pqmc_set_precision(pqmc_handle *h, int mode);
 
+/* Householder QR of the stabilizations (ZGEQRF + ZUNGQR in qdr / ldq,
+ * src/mod_matrixlib.f90:323-360): 0 = one launch per factorization, a
+ * workgroup per 16-column panel, each panel's reflector handed to the panels
+ * right of it through flags (default); 1 = a panel launch and a
+ * block-reflector launch per panel.  Same conventions and outputs, equal to
+ * rounding.  No reference counterpart (LAPACK's blocking is internal). */
+int pqmc_set_qr_mode(pqmc_handle *h, int mode);
+
 /* init_MC: init_expKV + set_pf_expKV_mat + init_g(start_time)
  * (src/Main_PQMC.f90:185-203, src/mod_evolution.f90:46-111). */
 int pqmc_init_g(pqmc_handle *h, int start_time);

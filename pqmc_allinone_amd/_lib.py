@@ -52,6 +52,7 @@ SIGNATURES = {
     "pqmc_set_trial": (_I, [_P, _P]),
     "pqmc_init_g": (_I, [_P, _I]),
     "pqmc_set_precision": (_I, [_P, _I]),
+    "pqmc_set_qr_mode": (_I, [_P, _I]),
     "pqmc_update_family": (_I, [_P, _I, _I, _P, _P]),
     "pqmc_evolve_g": (_I, [_P, _I, _I, _I]),
     "pqmc_evolve_matrix": (_I, [_P, _I, _I, _I, _I, _P, _I]),

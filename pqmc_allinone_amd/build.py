@@ -9,7 +9,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["pqmc_abi.hip", "k_pass.hip", "k_wrap.hip", "k_dense.hip", "k_meas.hip", "k_mixed.hip"]
+SOURCES = ["pqmc_abi.hip", "k_pass.hip", "k_wrap.hip", "k_dense.hip", "k_meas.hip", "k_mixed.hip", "k_qrflow.hip"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wno-unused-value",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

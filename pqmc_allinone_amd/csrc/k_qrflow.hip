@@ -1,0 +1,636 @@
+// One-launch blocked Householder QR of A (m x n, n <= m <= 1024) for the
+// stabilization's QDR factorizations (qdr, src/mod_matrixlib.f90:323-334 and
+// :452-467: ZGEQRF then ZUNGQR).  Outputs as the panel-by-panel form
+// (k_qr_panel + k_apply_refl): R in A's upper triangle with beta on the
+// diagonal and the reflectors below it, V (unit diagonal, zeros above) in its
+// own array, T of each panel's block reflector (16 x 16, ld 16, zero padded),
+// tau, beta and 1 + |x|^2 |1/(alpha-beta)|^2 = v^H v per column, and Q
+// (m x n) explicit when asked for.  zgeqr2 / zlarfg / zlarft conventions:
+// beta = -sign(Re alpha) |(alpha, x)|, tau = ((beta - Re alpha)/beta,
+// -Im alpha/beta), v = x / (alpha - beta); H^H = I - conj(tau) v v^H is
+// applied from the left.
+//
+// Workgroup b owns panel b, columns [b nb, b nb + nb), and keeps them in LDS
+// for the whole launch:
+//  1. for each p < b, it waits for panel p's flag and applies
+//     H_p^H = I - V_p T_p^H V_p^H to its columns (FP64 MFMA);
+//  2. it factors its panel, one wave per column and one barrier per column:
+//     every wave forms the column's norm and its own product with the column
+//     and derives the reflector itself, so no wave waits on another's scalars;
+//  3. it publishes V_b and T_b with device-scope stores, waits for them, and
+//     raises panel b's flag;
+//  4. it writes its columns of A, then forms Q's columns [b nb, b nb + nb) as
+//     H_0 H_1 ... H_b E_b (H_p with p > b leaves them unchanged).
+// The critical path is the chain of panels (apply the previous reflector,
+// factor, publish); the trailing updates of later panels and the formation of
+// Q run beside it.
+//
+// A workgroup takes its panel from an atomic ticket when it starts, so panel
+// b waits only on panels whose workgroups are already running: the launch
+// completes whatever else holds the compute units, with no co-residency
+// requirement.  Flags hold the launch's epoch, so nothing is cleared between
+// launches.  Waits are bounded (0.5 s of wall clock); an expired wait marks
+// the launch aborted, counts itself in *timeouts and ends every workgroup.
+//
+// Cross-XCD visibility follows the family pass: V and T are written only with
+// device-scope stores and read only after their flag, each address once per
+// launch, and the caches are invalidated at kernel start, so plain L2-cached
+// loads serve the readers.
+#include <mutex>
+
+#include "k_qrflow.h"
+
+#define QF_THREADS 1024
+#define QF_WAVES (QF_THREADS / 64)
+#define QF_WAIT_TICKS 50000000ull  // 0.5 s of the 100 MHz wall clock
+#define QF_PART 4352               // bytes of one wave's partial 16 x 16 tile (re and im, ld 17)
+
+static __device__ __forceinline__ unsigned long long qf_ld(unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ void qf_st(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ void qf_st_z(zd *p, zd v) {
+  double *d = reinterpret_cast<double *>(p);
+  __hip_atomic_store(d, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(d + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait (all threads) until panel p's flag holds this launch's epoch.
+// Returns false when the launch is aborted.
+static __device__ bool qf_wait(const qf_args &a, int p, int *sStop) {
+  if (threadIdx.x == 0) {
+    unsigned long long *f = &a.sync[2 + p];
+    const unsigned long long t0 = wall_clock64();
+    int st = 0;
+    for (unsigned it = 1;; ++it) {
+      if (qf_ld(f) >= a.epoch) break;
+      if ((it & 31) == 0) {
+        if (qf_ld(&a.sync[1]) == a.epoch) {
+          st = 1;
+          break;
+        }
+        if (wall_clock64() - t0 > QF_WAIT_TICKS) {
+          qf_st(&a.sync[1], a.epoch);
+          atomicAdd(a.timeouts, 1ull);
+          st = 1;
+          break;
+        }
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *sStop = st;
+  }
+  __syncthreads();
+  return *sStop == 0;
+}
+
+// Rows [r0, m) of one LDS column times s, eight rows per lane at a time
+// (one wave).
+static __device__ __forceinline__ void qf_scale_col(zd *col, int r0, int m, zd s) {
+  const int lane = threadIdx.x & 63;
+  for (int base = r0 + lane; base < m; base += 8 * 64) {
+    zd v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = base + 64 * i;
+      v[i] = r < m ? col[r] : zmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = base + 64 * i;
+      if (r < m) col[r] = zmul(v[i], s);
+    }
+  }
+}
+
+// sA (m x nb, ld m, the workgroup's ncol columns) <- H_p^H sA (trans = 1) or
+// H_p sA (trans = 0) on rows [p nb, m), H_p = I - V_p T_p V_p^H:
+//   W = V_p^H sA  (npart waves split the rows, four 4-row chunks per wave
+//                  in flight; partial tiles in sPart, then summed into sW),
+//   W2 = op(T_p) W  (into sPart),
+//   sA -= V_p W2  (16-row tiles over all waves).
+static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
+                                zd *sTp, int p, int ncol, int trans) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m = a.m, nb = a.nb;
+  const int r0 = p * nb;
+  const int npw = min(nb, a.n - r0);  // width of panel p
+  const zd *Vp = a.V + (size_t)m * r0;
+  const zd *Tp = a.T + 256 * (size_t)p;
+  const bool vk = li < npw, ac = li < ncol;
+  const int npart = a.npart;
+  if (tid < 256) sTp[tid] = Tp[tid];  // T_p, read after the barriers below
+  if (wv < npart) {
+    double4_t cr0 = {0.0, 0.0, 0.0, 0.0}, ci0 = cr0, cr1 = cr0, ci1 = cr0;
+    // chunk q of wave wv at rows r0 + 4 (wv + npart q)
+    for (int rb = r0 + 4 * wv; rb < m; rb += 16 * npart) {
+      zd vv[4], xx[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = rb + 4 * npart * u + lk;
+        vv[u] = (r < m && vk) ? Vp[r + (size_t)m * li] : zmk(0.0, 0.0);
+        xx[u] = (r < m && ac) ? sA[r + m * li] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u += 2) {
+        // conj(V)^T sA: re = Vr Ar + Vi Ai, im = Vr Ai - Vi Ar
+        cr0 = mfma_f64(vv[u].x, xx[u].x, cr0);
+        ci0 = mfma_f64(vv[u].x, xx[u].y, ci0);
+        cr1 = mfma_f64(vv[u + 1].x, xx[u + 1].x, cr1);
+        ci1 = mfma_f64(vv[u + 1].x, xx[u + 1].y, ci1);
+        cr0 = mfma_f64(vv[u].y, xx[u].y, cr0);
+        ci0 = mfma_f64(-vv[u].y, xx[u].x, ci0);
+        cr1 = mfma_f64(vv[u + 1].y, xx[u + 1].y, cr1);
+        ci1 = mfma_f64(-vv[u + 1].y, xx[u + 1].x, ci1);
+      }
+    }
+    double *pr = sPart + (size_t)wv * (QF_PART / 8), *pi = pr + 16 * 17;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      pr[(lk + 4 * i) * 17 + li] = cr0[i] + cr1[i];
+      pi[(lk + 4 * i) * 17 + li] = ci0[i] + ci1[i];
+    }
+  }
+  __syncthreads();
+  if (tid < 256) {  // W(k, cc): the partial tiles summed, all loads first
+    const int k = tid >> 4, cc = tid & 15;
+    double xr[8], xi[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      const double *pw = sPart + (size_t)w * (QF_PART / 8);
+      xr[w] = w < npart ? pw[k * 17 + cc] : 0.0;
+      xi[w] = w < npart ? pw[16 * 17 + k * 17 + cc] : 0.0;
+    }
+    sWr[k][cc] = ((xr[0] + xr[1]) + (xr[2] + xr[3])) + ((xr[4] + xr[5]) + (xr[6] + xr[7]));
+    sWi[k][cc] = ((xi[0] + xi[1]) + (xi[2] + xi[3])) + ((xi[4] + xi[5]) + (xi[6] + xi[7]));
+  }
+  __syncthreads();
+  double *w2 = sPart;  // W2, re then im, ld 17
+  if (tid < 256) {
+    const int kk = tid >> 4, cc = tid & 15;
+    double wr = 0.0, wi = 0.0;
+    // T is zero outside the panel's npw x npw block and W outside its
+    // npw x ncol block, so the full 16-term sums are the products
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      zd tk[4];
+      double wr_[4], wi_[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        tk[k] = trans ? zconj(sTp[k0 + k + 16 * kk]) : sTp[kk + 16 * (k0 + k)];
+        wr_[k] = sWr[k0 + k][cc];
+        wi_[k] = sWi[k0 + k][cc];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        wr += tk[k].x * wr_[k] - tk[k].y * wi_[k];
+        wi += tk[k].x * wi_[k] + tk[k].y * wr_[k];
+      }
+    }
+    w2[kk * 17 + cc] = (kk < npw && cc < ncol) ? wr : 0.0;
+    w2[16 * 17 + kk * 17 + cc] = (kk < npw && cc < ncol) ? wi : 0.0;
+  }
+  __syncthreads();
+  const int nk4 = (npw + 3) >> 2;
+  double w2r[4], w2i[4];
+#pragma unroll
+  for (int k4 = 0; k4 < 4; ++k4) {
+    w2r[k4] = w2[(4 * k4 + lk) * 17 + li];
+    w2i[k4] = w2[16 * 17 + (4 * k4 + lk) * 17 + li];
+  }
+  // 16-row tiles t0 = r0 + 16 (wv + 16 q)
+  for (int tb = r0 + 16 * wv; tb < m; tb += 16 * QF_WAVES) {
+    zd vv[1][4], aa[1][4];
+#pragma unroll
+    for (int q = 0; q < 1; ++q) {
+      const int t0 = tb + 16 * QF_WAVES * q;
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const int r = t0 + li, kc = 4 * k4 + lk;
+        vv[q][k4] = (k4 < nk4 && r < m && kc < npw) ? Vp[r + (size_t)m * kc] : zmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = t0 + lk + 4 * i;
+        aa[q][i] = (row < m && ac) ? sA[row + m * li] : zmk(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 1; ++q) {
+      const int t0 = tb + 16 * QF_WAVES * q;
+      if (t0 < m) {  // uniform
+        double4_t pr = {0.0, 0.0, 0.0, 0.0}, pi = pr;
+#pragma unroll
+        for (int k4 = 0; k4 < 4; ++k4) {
+          if (k4 < nk4) {  // uniform
+            // (Vr + i Vi)(Wr + i Wi)
+            pr = mfma_f64(vv[q][k4].x, w2r[k4], pr);
+            pi = mfma_f64(vv[q][k4].x, w2i[k4], pi);
+            pr = mfma_f64(-vv[q][k4].y, w2i[k4], pr);
+            pi = mfma_f64(vv[q][k4].y, w2r[k4], pi);
+          }
+        }
+        if (ac) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = t0 + lk + 4 * i;
+            if (row < m) sA[row + m * li] = zsub(aa[q][i], zmk(pr[i], pi[i]));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Factor the panel in sA (columns 0..nbp-1, rows [j0, m)), one wave per
+// column, every column in LDS (any m).  The pivot wave scales its column
+// into the reflector only at the next step, once no wave reads it.  Rows go
+// in batches of eight per lane, every load of a batch issued before its
+// arithmetic.
+static __device__ void qf_factor_lds(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  zd sck = zmk(0.0, 0.0);  // the pivot wave's 1 / (alpha - beta), kept for that store
+  for (int j = 0; j < nbp; ++j) {
+    const int dg = j0 + j;
+    if (wv == j - 1) qf_scale_col(sA + (size_t)m * (j - 1), dg, m, sck);
+    if (wv >= j && wv < nbp) {  // uniform per wave
+      const int c = wv;
+      const zd *xc = sA + (size_t)m * j;
+      zd *yc = sA + (size_t)m * c;
+      double nx0 = 0.0, nx1 = 0.0, dr0 = 0.0, dr1 = 0.0, di0 = 0.0, di1 = 0.0;
+      for (int base = dg + 1 + lane; base < m; base += 8 * 64) {
+        zd xs[8], ys[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int r = base + 64 * i;
+          xs[i] = r < m ? xc[r] : zmk(0.0, 0.0);
+          ys[i] = r < m ? yc[r] : zmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+          nx0 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
+          nx1 += xs[i + 1].x * xs[i + 1].x + xs[i + 1].y * xs[i + 1].y;
+          dr0 += xs[i].x * ys[i].x + xs[i].y * ys[i].y;  // conj(x) y
+          dr1 += xs[i + 1].x * ys[i + 1].x + xs[i + 1].y * ys[i + 1].y;
+          di0 += xs[i].x * ys[i].y - xs[i].y * ys[i].x;
+          di1 += xs[i + 1].x * ys[i + 1].y - xs[i + 1].y * ys[i + 1].x;
+        }
+      }
+      double nx = nx0 + nx1, dr = dr0 + dr1, di = di0 + di1;
+      wave_sum3(nx, dr, di);
+      const zd alpha = xc[dg];
+      zd t, sc, bt;
+      double v2;
+      if (nx == 0.0 && alpha.y == 0.0) {
+        t = zmk(0.0, 0.0);
+        sc = zmk(0.0, 0.0);
+        bt = alpha;
+        v2 = 1.0;
+      } else {
+        const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + nx), alpha.x);
+        const double rb = 1.0 / beta;
+        t = zmk((beta - alpha.x) * rb, -alpha.y * rb);
+        // sc = 1 / (alpha - beta) = conj(alpha - beta) / |alpha - beta|^2
+        const double er = alpha.x - beta, rd = 1.0 / (er * er + alpha.y * alpha.y);
+        sc = zmk(er * rd, -alpha.y * rd);
+        bt = zmk(beta, 0.0);
+        v2 = 1.0 + nx * rd;
+      }
+      if (c == j) {
+        sck = sc;
+        if (lane == 0) {
+          sTau[j] = t;
+          sBeta[j] = bt;
+          sVn2[j] = v2;
+        }
+      } else {
+        // w_c = tau^* (a_{dg,c} + sc^* x^H a_c); a_c -= v w_c with v = (1, x sc)
+        const zd acd = yc[dg];
+        const zd wc = zmul(zconj(t), zadd(acd, zmul(zconj(sc), zmk(dr, di))));
+        const zd f = zmul(sc, wc);
+        if (lane == 0) yc[dg] = zsub(acd, wc);
+        for (int base = dg + 1 + lane; base < m; base += 8 * 64) {
+          zd xs[8], ys[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int r = base + 64 * i;
+            xs[i] = r < m ? xc[r] : zmk(0.0, 0.0);
+            ys[i] = r < m ? yc[r] : zmk(0.0, 0.0);
+          }
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int r = base + 64 * i;
+            if (r < m) yc[r] = zsub(ys[i], zmul(xs[i], f));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (wv == nbp - 1) qf_scale_col(sA + (size_t)m * (nbp - 1), j0 + nbp, m, sck);
+  __syncthreads();
+}
+
+// The same with each wave's column in registers, eight rows per lane
+// (m <= 512): per step a wave reads only the pivot column from LDS, and the
+// next pivot stores its updated column there for the step after.
+static __device__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool own = wv < nbp;
+  zd col[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = lane + 64 * i;
+    col[i] = (own && r < m) ? sA[r + (size_t)m * wv] : zmk(0.0, 0.0);
+  }
+  for (int j = 0; j < nbp; ++j) {
+    const int dg = j0 + j;
+    if (own && wv >= j) {  // uniform per wave
+      const zd *xc = sA + (size_t)m * j;
+      zd xs[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int r = lane + 64 * i;
+        xs[i] = (r > dg && r < m) ? xc[r] : zmk(0.0, 0.0);
+      }
+      double nx0 = 0.0, nx1 = 0.0, dr0 = 0.0, dr1 = 0.0, di0 = 0.0, di1 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 8; i += 2) {
+        nx0 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
+        nx1 += xs[i + 1].x * xs[i + 1].x + xs[i + 1].y * xs[i + 1].y;
+        dr0 += xs[i].x * col[i].x + xs[i].y * col[i].y;  // conj(x) y
+        dr1 += xs[i + 1].x * col[i + 1].x + xs[i + 1].y * col[i + 1].y;
+        di0 += xs[i].x * col[i].y - xs[i].y * col[i].x;
+        di1 += xs[i + 1].x * col[i + 1].y - xs[i + 1].y * col[i + 1].x;
+      }
+      double nx = nx0 + nx1, dr = dr0 + dr1, di = di0 + di1;
+      wave_sum3(nx, dr, di);
+      const zd alpha = xc[dg];
+      zd t, sc, bt;
+      double v2;
+      if (nx == 0.0 && alpha.y == 0.0) {
+        t = zmk(0.0, 0.0);
+        sc = zmk(0.0, 0.0);
+        bt = alpha;
+        v2 = 1.0;
+      } else {
+        const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + nx), alpha.x);
+        const double rb = 1.0 / beta;
+        t = zmk((beta - alpha.x) * rb, -alpha.y * rb);
+        const double er = alpha.x - beta, rd = 1.0 / (er * er + alpha.y * alpha.y);
+        sc = zmk(er * rd, -alpha.y * rd);
+        bt = zmk(beta, 0.0);
+        v2 = 1.0 + nx * rd;
+      }
+      if (wv == j) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int r = lane + 64 * i;
+          if (r > dg && r < m) col[i] = zmul(col[i], sc);
+        }
+        if (lane == 0) {
+          sTau[j] = t;
+          sBeta[j] = bt;
+          sVn2[j] = v2;
+        }
+      } else {
+        // a_{dg,c}: this wave's register of row dg (lane dg % 64, slot dg / 64)
+        zd adg = zmk(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (i == (dg >> 6)) adg = col[i];
+        const zd acd = zmk(__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(adg.x), dg & 63),
+                                            __builtin_amdgcn_readlane(__double2loint(adg.x), dg & 63)),
+                           __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(adg.y), dg & 63),
+                                            __builtin_amdgcn_readlane(__double2loint(adg.y), dg & 63)));
+        const zd wc = zmul(zconj(t), zadd(acd, zmul(zconj(sc), zmk(dr, di))));
+        const zd f = zmul(sc, wc);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int r = lane + 64 * i;
+          if (r == dg) col[i] = zsub(col[i], wc);
+          else if (r > dg && r < m) col[i] = zsub(col[i], zmul(xs[i], f));
+        }
+        if (wv == j + 1) {  // the next pivot column, for every wave's next step
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int r = lane + 64 * i;
+            if (r > dg && r < m) sA[r + (size_t)m * wv] = col[i];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (own) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = lane + 64 * i;
+      if (r < m) sA[r + (size_t)m * wv] = col[i];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
+  extern __shared__ zd qf_lds[];  // zd: 16-byte aligned after the static arrays
+  __shared__ double sW2r[16][17], sW2i[16][17];
+  __shared__ zd sTau[16], sBeta[16];
+  __shared__ double sVn2[16];
+  __shared__ zd sTp[256];
+  __shared__ int sIdx, sStop;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = a.m, nb = a.nb;
+  zd *sA = qf_lds;
+  double *sPart = reinterpret_cast<double *>(qf_lds + (size_t)m * nb);
+  if (tid == 0) {
+    const unsigned long long t =
+        __hip_atomic_fetch_add(&a.sync[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - a.ticket_base;
+    sIdx = t < (unsigned long long)a.np ? (int)t : -1;
+    sStop = 0;
+    if (sIdx < 0) {  // ticket count out of step with the host: abort the launch
+      qf_st(&a.sync[1], a.epoch);
+      atomicAdd(a.timeouts, 1ull);
+    }
+  }
+  __syncthreads();
+  const int b = sIdx;
+  if (b < 0) return;
+  const int j0 = b * nb, nbp = min(nb, a.n - j0);
+  // the panel's columns, rows [0, m); columns past the matrix are zero
+  for (int e = tid; e < m * nb; e += QF_THREADS) {
+    const int r = e % m, c = e / m;
+    sA[e] = c < nbp ? a.A[r + (size_t)a.lda * (j0 + c)] : zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  // 1. the reflectors of the panels to the left
+  for (int p = 0; p < b; ++p) {
+    if (!qf_wait(a, p, &sStop)) return;
+    qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 1);
+  }
+  // 2. factor the panel, one wave per column
+  if (m <= 512)
+    qf_factor_regs(sA, m, j0, nbp, sTau, sBeta, sVn2);
+  else
+    qf_factor_lds(sA, m, j0, nbp, sTau, sBeta, sVn2);
+  // 3. publish V_b (0 above row j0 + c, 1 on it, the reflector below it);
+  // these stores drain while T is formed
+  for (int e = tid; e < (m - j0) * nbp; e += QF_THREADS) {
+    const int r = j0 + e % (m - j0), c = e / (m - j0);
+    const zd v = (r < j0 + c) ? zmk(0.0, 0.0) : (r == j0 + c ? zmk(1.0, 0.0) : sA[r + m * c]);
+    qf_st_z(a.V + r + (size_t)m * (j0 + c), v);
+  }
+  // T of zlarft from the Gram matrix Z = V^H V (rows [j0, m))
+  const int npart = a.npart;
+  if (wv < npart) {
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t zr = {0.0, 0.0, 0.0, 0.0}, zi = zr;
+    for (int r0 = j0 + 4 * wv; r0 < m; r0 += 4 * npart) {
+      const int r = r0 + lk;
+      zd v = zmk(0.0, 0.0);
+      if (r < m && li < nbp) v = (r < j0 + li) ? zmk(0.0, 0.0) : (r == j0 + li ? zmk(1.0, 0.0) : sA[r + m * li]);
+      zr = mfma_f64(v.x, v.x, zr);
+      zi = mfma_f64(v.x, v.y, zi);
+      zr = mfma_f64(v.y, v.y, zr);
+      zi = mfma_f64(-v.y, v.x, zi);
+    }
+    double *pr = sPart + (size_t)wv * (QF_PART / 8), *pi = pr + 16 * 17;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      pr[(lk + 4 * i) * 17 + li] = zr[i];
+      pi[(lk + 4 * i) * 17 + li] = zi[i];
+    }
+  }
+  __syncthreads();
+  if (tid < 256) {  // Z(l, i) into sW2r / sW2i
+    const int l = tid >> 4, i = tid & 15;
+    double zr = 0.0, zi = 0.0;
+    for (int w = 0; w < npart; ++w) {
+      const double *pr = sPart + (size_t)w * (QF_PART / 8);
+      zr += pr[l * 17 + i];
+      zi += pr[16 * 17 + l * 17 + i];
+    }
+    sW2r[l][i] = zr;
+    sW2i[l][i] = zi;
+  }
+  __syncthreads();
+  // T^-1 = diag(1/tau) + striu(Z), so T = (I + N)^-1 diag(tau) with
+  // N = diag(tau) striu(Z) strictly upper (N^16 = 0) and
+  // (I + N)^-1 = (I - N)(I + N^2)(I + N^4)(I + N^8): six 16 x 16 products,
+  // one entry per thread (no 1/tau, so tau = 0 needs no care).
+  {
+    zd *sPw = reinterpret_cast<zd *>(sPart);  // N, then its powers
+    zd *sWk = sPw + 256;                       // the product so far
+    const int i = tid & 15, k = (tid >> 4) & 15;
+    const bool own = tid < 256;
+    if (own) {
+      const zd ti = i < nbp ? sTau[i] : zmk(0.0, 0.0);
+      const zd nik = (i < k && k < nbp) ? zmul(ti, zmk(sW2r[i][k], sW2i[i][k])) : zmk(0.0, 0.0);
+      sPw[i + 16 * k] = nik;
+      sWk[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
+    }
+    __syncthreads();
+    for (int s = 0; s < 3; ++s) {
+      zd pp = zmk(0.0, 0.0);
+      if (own) {
+#pragma unroll
+        for (int l0 = 0; l0 < 16; l0 += 8) {
+          zd x[8], y[8];
+#pragma unroll
+          for (int l = 0; l < 8; ++l) {
+            x[l] = sPw[i + 16 * (l0 + l)];
+            y[l] = sPw[l0 + l + 16 * k];
+          }
+#pragma unroll
+          for (int l = 0; l < 8; ++l) pp = zadd(pp, zmul(x[l], y[l]));
+        }
+      }
+      __syncthreads();
+      if (own) sPw[i + 16 * k] = pp;
+      __syncthreads();
+      zd wp = zmk(0.0, 0.0);
+      if (own) {
+#pragma unroll
+        for (int l0 = 0; l0 < 16; l0 += 8) {
+          zd x[8], y[8];
+#pragma unroll
+          for (int l = 0; l < 8; ++l) {
+            x[l] = sWk[i + 16 * (l0 + l)];
+            y[l] = sPw[l0 + l + 16 * k];
+          }
+#pragma unroll
+          for (int l = 0; l < 8; ++l) wp = zadd(wp, zmul(x[l], y[l]));
+        }
+      }
+      __syncthreads();
+      if (own) sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
+      __syncthreads();
+    }
+    // T_b, then the flag once every store of V_b and T_b is complete
+    if (own) qf_st_z(a.T + 256 * (size_t)b + i + 16 * k, (k < nbp && i < nbp) ? zmul(sWk[i + 16 * k], sTau[k]) : zmk(0.0, 0.0));
+  }
+  if (tid < nbp) {
+    a.tau[j0 + tid] = sTau[tid];
+    a.beta[j0 + tid] = sBeta[tid];
+    a.vn2[j0 + tid] = sVn2[tid];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+  __syncthreads();
+  if (tid == 0) qf_st(&a.sync[2 + b], a.epoch);
+  // 4. the panel's columns of A: R above the diagonal, beta on it, v below
+  for (int e = tid; e < m * nbp; e += QF_THREADS) {
+    const int r = e % m, c = e / m;
+    a.A[r + (size_t)a.lda * (j0 + c)] = (r == j0 + c) ? sBeta[c] : sA[e];
+  }
+  if (!a.Q) return;
+  __syncthreads();
+  // Q(:, j0 + c) = H_0 ... H_b e_{j0 + c}
+  for (int e = tid; e < m * nb; e += QF_THREADS) {
+    const int r = e % m, c = e / m;
+    sA[e] = zmk((c < nbp && r == j0 + c) ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int p = b; p >= 0; --p) qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 0);
+  for (int e = tid; e < m * nbp; e += QF_THREADS) {
+    const int r = e % m, c = e / m;
+    a.Q[r + (size_t)m * (j0 + c)] = sA[e];
+  }
+}
+
+extern "C" size_t pqmc_qr_flow_lds(int m, int nb, int *npart) {
+  const size_t base = sizeof(zd) * (size_t)m * nb;
+  const size_t budget = 160 * 1024 - 10 * 1024;  // less the kernel's static LDS
+  const int np = (base + 8 * QF_PART <= budget) ? 8 : 4;
+  if (npart) *npart = np;
+  return base + (size_t)np * QF_PART;
+}
+
+// Launch: a->np = ceil(n / nb) workgroups; the caller sets epoch (one more
+// than the last launch on this sync array) and ticket_base (the tickets
+// issued so far, np per launch).
+extern "C" hipError_t pqmc_qr_flow(qf_args *a, hipStream_t s) {
+  if (a->m > 1024 || a->n > a->m || a->n <= 0 || (a->nb != 8 && a->nb != 16) || a->np != (a->n + a->nb - 1) / a->nb ||
+      a->np > 128)
+    return hipErrorInvalidValue;
+  int npart = 0;
+  const size_t lds = pqmc_qr_flow_lds(a->m, a->nb, &npart);
+  if (lds > 160 * 1024 - 10 * 1024) return hipErrorInvalidValue;
+  a->npart = npart;
+  static std::mutex attr_mu;  // launches may come from several host threads
+  static size_t attr = 0;
+  {
+    std::lock_guard<std::mutex> lk(attr_mu);
+    if (lds > attr) {
+      hipError_t e = hipFuncSetAttribute((const void *)k_qr_flow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = lds;
+    }
+  }
+  hipLaunchKernelGGL(k_qr_flow, dim3(a->np), dim3(QF_THREADS), lds, s, *a);
+  return hipGetLastError();
+}

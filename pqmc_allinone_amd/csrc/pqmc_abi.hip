@@ -18,6 +18,7 @@
 #include "../../include/pqmc_hip.h"
 #include "pqmc_common.h"
 #include "k_pass.h"
+#include "k_qrflow.h"
 
 // ---- kernel launchers (k_*.hip) ----
 struct prop_args {
@@ -85,6 +86,10 @@ static int fail(int code, const std::string &msg) {
 struct qr_ws {  // workspace of one blocked QR
   zd *V = nullptr, *T = nullptr, *tau = nullptr, *beta = nullptr;
   double *vn2 = nullptr;
+  // one-launch form: ticket counter, abort word and panel flags; the host
+  // counts launches (epoch) and the tickets they took
+  unsigned long long *sync = nullptr;
+  unsigned long long epoch = 0, tickets = 0;
 };
 
 struct ev_pool {
@@ -157,6 +162,7 @@ struct pqmc_handle {
   // three-term split, 2 BF16x3 (k_mixed.hip); the dense group product P
   // (Ns x Ns), the product's output (Ns x ne) and the split scales
   int prec = 0;
+  int qr_mode = 0;  // 0: one-launch QR (k_qr_flow), 1: panel by panel (k_qr_panel + k_apply_refl)
   zd *d_mxP = nullptr, *d_mxX = nullptr;
   double *d_mxs = nullptr;
   qr_ws qf;              // QR workspace for Ns x Ns
@@ -301,6 +307,8 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->qm.tau, ne);
   A_(h->qm.beta, ne);
   A_(h->qm.vn2, ne);
+  A_(h->qa.sync, QF_SYNC_WORDS);
+  A_(h->qm.sync, QF_SYNC_WORDS);
   A_(h->d_acc, (size_t)h->Nb * h->Lt);
   A_(h->d_slot, pqmc_pass_slot_elems(Ns, h->nblk));
   A_(h->d_tslot, pqmc_pass_tslot_elems(h->nblk));
@@ -328,11 +336,18 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
     A_(h->qf.tau, Ns);
     A_(h->qf.beta, Ns);
     A_(h->qf.vn2, Ns);
+    A_(h->qf.sync, QF_SYNC_WORDS);
+    if (hipMemsetAsync(h->qf.sync, 0, QF_SYNC_WORDS * sizeof(unsigned long long), h->st) != hipSuccess) {
+      pqmc_destroy(h);
+      return fail(PQMC_EHIP, "initial memset failed");
+    }
   }
 #undef A_
   if (hipMemsetAsync(h->d_dlog, 0, h->nblock * sizeof(zd), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_psync, 0, pqmc_pass_sync_words(h->nblk) * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_stats, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
+      hipMemsetAsync(h->qa.sync, 0, QF_SYNC_WORDS * sizeof(unsigned long long), h->st) != hipSuccess ||
+      hipMemsetAsync(h->qm.sync, 0, QF_SYNC_WORDS * sizeof(unsigned long long), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_lncw, 0, sizeof(zd), h->st) != hipSuccess ||
       hipMemsetAsync(h->d_err, 0, 4 * sizeof(unsigned long long), h->st) != hipSuccess ||
@@ -373,7 +388,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
                   h->d_lncw, h->d_stats, h->d_err, h->d_hist, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
                   h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad,
-                  h->d_mxP, h->d_mxX, h->d_mxs};
+                  h->d_mxP, h->d_mxX, h->d_mxs, h->qa.sync, h->qm.sync, h->qf.sync};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (auto &pool : h->pools)
@@ -457,6 +472,31 @@ extern "C" int pqmc_get_fields(pqmc_handle *h, double *x) {
 static int run_qr(pqmc_handle *h, qr_ws &w, zd *A, int lda, int m, int n, zd *Q, zd *D, zd *Rout, int conjt) {
   const int nb = h->nbq;
   const int np = (n + nb - 1) / nb;
+  if (h->qr_mode == 0) {
+    // one launch: panel b waits for panels 0..b-1 (k_qrflow.hip)
+    qf_args a{};
+    a.A = A;
+    a.lda = lda;
+    a.m = m;
+    a.n = n;
+    a.nb = nb;
+    a.np = np;
+    a.Q = Q;
+    a.V = w.V;
+    a.T = w.T;
+    a.tau = w.tau;
+    a.beta = w.beta;
+    a.vn2 = w.vn2;
+    a.sync = w.sync;
+    a.timeouts = h->d_stats + 2;
+    a.epoch = w.epoch + 1;
+    a.ticket_base = w.tickets;
+    HIPCHK(pqmc_qr_flow(&a, h->st));
+    w.epoch += 1;
+    w.tickets += np;
+    if (D && Rout) HIPCHK(pqmc_qdr_finish(A, lda, n, Rout, D, conjt, h->st));
+    return PQMC_OK;
+  }
   for (int p = 0; p < np; ++p) {
     const int j0 = p * nb, nbp = std::min(nb, n - j0);
     HIPCHK(pqmc_qr_panel(A, lda, m, j0, nbp, w.V, m, w.T + (size_t)256 * p, w.tau + j0, w.vn2 + j0, w.beta + j0,
@@ -550,6 +590,14 @@ extern "C" int pqmc_set_precision(pqmc_handle *h, int mode) {
     HIPCHK(dalloc(&h->d_mxs, Ns + h->ne));
   }
   h->prec = mode;
+  return PQMC_OK;
+}
+
+extern "C" int pqmc_set_qr_mode(pqmc_handle *h, int mode) {
+  if (!h) return fail(PQMC_EINVAL, "null handle");
+  if (mode < 0 || mode > 1) return fail(PQMC_EINVAL, "QR mode must be 0 (one launch) or 1 (panel by panel)");
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->qr_mode = mode;
   return PQMC_OK;
 }
 
@@ -745,7 +793,10 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   if (h->ft) {
     int rc = ft_init(h);
     if (rc) return rc;
+    unsigned long long stats[4];
+    HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a QR");
     h->g_ready = true;
     h->g_at = 1;
     return PQMC_OK;
@@ -781,7 +832,10 @@ extern "C" int pqmc_init_g(pqmc_handle *h, int start_time) {
   }
   int rc = cal_g(h, QS(h, nb), rmat, h->d_G, false);
   if (rc) return rc;
+  unsigned long long stats[4];
+  HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a QR");
   h->g_ready = true;
   h->g_at = 1;
   return PQMC_OK;
@@ -964,10 +1018,12 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
 // the largest G or Re ln_cw drift, as get_g_scratch_T0 does
 // (src/mod_evolution.f90:179-193); a non-finite one fails the call.
 static int read_err(pqmc_handle *h, double *err) {
-  unsigned long long bits[4] = {0, 0, 0, 0};
+  unsigned long long bits[4] = {0, 0, 0, 0}, stats[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(bits, h->d_err, sizeof(bits), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipMemsetAsync(h->d_err + 2, 0, 2 * sizeof(unsigned long long), h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass or a QR");
   double e[4];
   std::memcpy(e, bits, sizeof(e));
   if (err) *err = e[0];
@@ -1048,7 +1104,7 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   unsigned long long stats[4];
   HIPCHK(hipMemcpyAsync(stats, h->d_stats, sizeof(stats), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass");
+  if (stats[2]) return fail(PQMC_ESYNC, "device-side wait expired in a family pass or a QR");
   h->g_at = ntc == Lt ? (forward ? 1 : 2) : 0;
   return PQMC_OK;
 }

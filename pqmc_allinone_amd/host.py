@@ -333,6 +333,16 @@ class Chain:
         init_g / get_g_scratch / sweep."""
         _lib.check(self._L.pqmc_set_precision(self.h, self.PRECISIONS[mode]))
 
+    QR_MODES = {"flow": 0, "panels": 1}
+
+    def set_qr_mode(self, mode: str):
+        """Householder QR of the stabilizations as one launch ("flow",
+        default: a workgroup per panel, panels handed on by flags) or panel
+        by panel ("panels": a panel launch and a block-reflector launch per
+        16 columns).  Same conventions and outputs; the results differ only
+        by rounding."""
+        _lib.check(self._L.pqmc_set_qr_mode(self.h, self.QR_MODES[mode]))
+
     def drift_history(self) -> np.ndarray:
         """(n, 2): G and Re ln_cw drift of each stabilization of the latest
         sweep, gradient pass or get_g_scratch, in order."""
