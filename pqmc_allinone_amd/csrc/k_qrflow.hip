@@ -12,13 +12,14 @@
 //
 // Workgroup b owns panel b, columns [b nb, b nb + nb), and keeps them in LDS
 // for the whole launch:
-//  1. for each p < b, it waits for panel p's flag and applies
-//     H_p^H = I - V_p T_p^H V_p^H to its columns (FP64 MFMA);
+//  1. for each p < b, it applies H_p^H = I - V_p T_p^H V_p^H to its columns
+//     (FP64 MFMA): V_p^H A once panel p's V flag is up, the rest once its T
+//     flag is;
 //  2. it factors its panel, one wave per column and one barrier per column:
 //     every wave forms the column's norm and its own product with the column
 //     and derives the reflector itself, so no wave waits on another's scalars;
-//  3. it publishes V_b and T_b with device-scope stores, waits for them, and
-//     raises panel b's flag;
+//  3. it publishes V_b with device-scope stores, waits for them and raises
+//     the V flag, then forms T_b and does the same for it;
 //  4. it writes its columns of A, then forms Q's columns [b nb, b nb + nb) as
 //     H_0 H_1 ... H_b E_b (H_p with p > b leaves them unchanged).
 // The critical path is the chain of panels (apply the previous reflector,
@@ -57,11 +58,19 @@ static __device__ __forceinline__ void qf_st_z(zd *p, zd v) {
   __hip_atomic_store(d + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wait (all threads) until panel p's flag holds this launch's epoch.
-// Returns false when the launch is aborted.
-static __device__ bool qf_wait(const qf_args &a, int p, int *sStop) {
+// v from lane l (uniform) to every lane
+static __device__ __forceinline__ zd rdlane_zq(zd v, int l) {
+  return zmk(__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.x), l),
+                              __builtin_amdgcn_readlane(__double2loint(v.x), l)),
+             __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v.y), l),
+                              __builtin_amdgcn_readlane(__double2loint(v.y), l)));
+}
+
+// Wait (all threads) until flag word w holds this launch's epoch.  Returns
+// false when the launch is aborted.
+static __device__ bool qf_wait(const qf_args &a, int w, int *sStop) {
   if (threadIdx.x == 0) {
-    unsigned long long *f = &a.sync[2 + p];
+    unsigned long long *f = &a.sync[w];
     const unsigned long long t0 = wall_clock64();
     int st = 0;
     for (unsigned it = 1;; ++it) {
@@ -107,32 +116,34 @@ static __device__ __forceinline__ void qf_scale_col(zd *col, int r0, int m, zd s
 
 // sA (m x nb, ld m, the workgroup's ncol columns) <- H_p^H sA (trans = 1) or
 // H_p sA (trans = 0) on rows [p nb, m), H_p = I - V_p T_p V_p^H:
-//   W = V_p^H sA  (npart waves split the rows, four 4-row chunks per wave
-//                  in flight; partial tiles in sPart, then summed into sW),
+//   W = V_p^H sA  (2 npart waves split the rows, four 4-row chunks per wave
+//                  in flight; the upper waves' partial tiles are folded into
+//                  the lower ones' in sPart, then summed into sW),
 //   W2 = op(T_p) W  (into sPart),
 //   sA -= V_p W2  (16-row tiles over all waves).
-static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
-                                zd *sTp, int p, int ncol, int trans) {
+// V_p must be published; T_p is waited for (flag word tflag, none if < 0)
+// after the first product.  Returns false when the launch is aborted.
+static __device__ bool qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
+                                zd *sTp, int p, int ncol, int trans, int tflag, int *sStop) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int m = a.m, nb = a.nb;
+  const int m = a.m, nb = a.nb, lm = m | 1;
   const int r0 = p * nb;
   const int npw = min(nb, a.n - r0);  // width of panel p
   const zd *Vp = a.V + (size_t)m * r0;
   const zd *Tp = a.T + 256 * (size_t)p;
   const bool vk = li < npw, ac = li < ncol;
-  const int npart = a.npart;
-  if (tid < 256) sTp[tid] = Tp[tid];  // T_p, read after the barriers below
-  if (wv < npart) {
+  const int npart = a.npart, nw = min(2 * npart, QF_WAVES);
+  if (wv < nw) {
     double4_t cr0 = {0.0, 0.0, 0.0, 0.0}, ci0 = cr0, cr1 = cr0, ci1 = cr0;
-    // chunk q of wave wv at rows r0 + 4 (wv + npart q)
-    for (int rb = r0 + 4 * wv; rb < m; rb += 16 * npart) {
+    // chunk q of wave wv at rows r0 + 4 (wv + nw q)
+    for (int rb = r0 + 4 * wv; rb < m; rb += 16 * nw) {
       zd vv[4], xx[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int r = rb + 4 * npart * u + lk;
+        const int r = rb + 4 * nw * u + lk;
         vv[u] = (r < m && vk) ? Vp[r + (size_t)m * li] : zmk(0.0, 0.0);
-        xx[u] = (r < m && ac) ? sA[r + m * li] : zmk(0.0, 0.0);
+        xx[u] = (r < m && ac) ? sA[r + lm * li] : zmk(0.0, 0.0);
       }
 #pragma unroll
       for (int u = 0; u < 4; u += 2) {
@@ -147,14 +158,36 @@ static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double 
         ci1 = mfma_f64(-vv[u + 1].y, xx[u + 1].x, ci1);
       }
     }
-    double *pr = sPart + (size_t)wv * (QF_PART / 8), *pi = pr + 16 * 17;
+    cr0 += cr1;
+    ci0 += ci1;
+    const int slot = wv < npart ? wv : wv - npart;
+    double *pr = sPart + (size_t)slot * (QF_PART / 8), *pi = pr + 16 * 17;
+    if (wv >= npart) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      pr[(lk + 4 * i) * 17 + li] = cr0[i] + cr1[i];
-      pi[(lk + 4 * i) * 17 + li] = ci0[i] + ci1[i];
+      for (int i = 0; i < 4; ++i) {
+        pr[(lk + 4 * i) * 17 + li] = cr0[i];
+        pi[(lk + 4 * i) * 17 + li] = ci0[i];
+      }
     }
+    __syncthreads();
+    if (wv < npart) {
+      const bool fold = wv + npart < nw;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int e = (lk + 4 * i) * 17 + li;
+        pr[e] = fold ? cr0[i] + pr[e] : cr0[i];
+        pi[e] = fold ? ci0[i] + pi[e] : ci0[i];
+      }
+    }
+  } else {
+    __syncthreads();
   }
-  __syncthreads();
+  if (tflag >= 0) {
+    if (!qf_wait(a, tflag, sStop)) return false;  // its barrier also closes the fold
+  } else {
+    __syncthreads();
+  }
+  if (tid < 256) sTp[tid] = Tp[tid];
   if (tid < 256) {  // W(k, cc): the partial tiles summed, all loads first
     const int k = tid >> 4, cc = tid & 15;
     double xr[8], xi[8];
@@ -168,7 +201,7 @@ static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double 
     sWi[k][cc] = ((xi[0] + xi[1]) + (xi[2] + xi[3])) + ((xi[4] + xi[5]) + (xi[6] + xi[7]));
   }
   __syncthreads();
-  double *w2 = sPart;  // W2, re then im, ld 17
+  double *w2 = sPart;  // W2, re then im, ld 17 (the partials are consumed)
   if (tid < 256) {
     const int kk = tid >> 4, cc = tid & 15;
     double wr = 0.0, wi = 0.0;
@@ -215,7 +248,7 @@ static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = t0 + lk + 4 * i;
-        aa[q][i] = (row < m && ac) ? sA[row + m * li] : zmk(0.0, 0.0);
+        aa[q][i] = (row < m && ac) ? sA[row + lm * li] : zmk(0.0, 0.0);
       }
     }
 #pragma unroll
@@ -237,13 +270,14 @@ static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double 
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int row = t0 + lk + 4 * i;
-            if (row < m) sA[row + m * li] = zsub(aa[q][i], zmk(pr[i], pi[i]));
+            if (row < m) sA[row + lm * li] = zsub(aa[q][i], zmk(pr[i], pi[i]));
           }
         }
       }
     }
   }
   __syncthreads();
+  return true;
 }
 
 // Factor the panel in sA (columns 0..nbp-1, rows [j0, m)), one wave per
@@ -252,15 +286,15 @@ static __device__ void qf_apply(const qf_args &a, zd *sA, double *sPart, double 
 // in batches of eight per lane, every load of a batch issued before its
 // arithmetic.
 static __device__ void qf_factor_lds(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lm = m | 1;
   zd sck = zmk(0.0, 0.0);  // the pivot wave's 1 / (alpha - beta), kept for that store
   for (int j = 0; j < nbp; ++j) {
     const int dg = j0 + j;
-    if (wv == j - 1) qf_scale_col(sA + (size_t)m * (j - 1), dg, m, sck);
+    if (wv == j - 1) qf_scale_col(sA + (size_t)lm * (j - 1), dg, m, sck);
     if (wv >= j && wv < nbp) {  // uniform per wave
       const int c = wv;
-      const zd *xc = sA + (size_t)m * j;
-      zd *yc = sA + (size_t)m * c;
+      const zd *xc = sA + (size_t)lm * j;
+      zd *yc = sA + (size_t)lm * c;
       double nx0 = 0.0, nx1 = 0.0, dr0 = 0.0, dr1 = 0.0, di0 = 0.0, di1 = 0.0;
       for (int base = dg + 1 + lane; base < m; base += 8 * 64) {
         zd xs[8], ys[8];
@@ -331,98 +365,112 @@ static __device__ void qf_factor_lds(zd *sA, int m, int j0, int nbp, zd *sTau, z
     }
     __syncthreads();
   }
-  if (wv == nbp - 1) qf_scale_col(sA + (size_t)m * (nbp - 1), j0 + nbp, m, sck);
+  if (wv == nbp - 1) qf_scale_col(sA + (size_t)lm * (nbp - 1), j0 + nbp, m, sck);
   __syncthreads();
 }
 
+// Reflector of the column held in registers (rows lane + 64 i) at diagonal
+// row dg (one wave): zlarfg's scalars from |x|^2 (rows > dg) and alpha; the
+// rows below dg scaled into v; v stored to the LDS column and tau, beta and
+// v^H v to the panel's scalars (slot j).
+static __device__ __forceinline__ void qf_reflector(zd (&col)[8], zd *scol, int m, int dg, int j, zd *sTau,
+                                                    zd *sBeta, double *sVn2) {
+  const int lane = threadIdx.x & 63;
+  double n0 = 0.0, n1 = 0.0;
+  zd adg = zmk(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < 8; i += 2) {
+    const int r = lane + 64 * i, r1 = r + 64;
+    if (r > dg && r < m) n0 += col[i].x * col[i].x + col[i].y * col[i].y;
+    if (r1 > dg && r1 < m) n1 += col[i + 1].x * col[i + 1].x + col[i + 1].y * col[i + 1].y;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (i == (dg >> 6)) adg = col[i];
+  const double nx = wave_sum(n0 + n1);
+  const zd alpha = rdlane_zq(adg, dg & 63);
+  zd t, sc, bt;
+  double v2;
+  if (nx == 0.0 && alpha.y == 0.0) {
+    t = zmk(0.0, 0.0);
+    sc = zmk(0.0, 0.0);
+    bt = alpha;
+    v2 = 1.0;
+  } else {
+    const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + nx), alpha.x);
+    const double rb = 1.0 / beta;
+    t = zmk((beta - alpha.x) * rb, -alpha.y * rb);
+    // sc = 1 / (alpha - beta) = conj(alpha - beta) / |alpha - beta|^2
+    const double er = alpha.x - beta, rd = 1.0 / (er * er + alpha.y * alpha.y);
+    sc = zmk(er * rd, -alpha.y * rd);
+    bt = zmk(beta, 0.0);
+    v2 = 1.0 + nx * rd;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = lane + 64 * i;
+    if (r > dg && r < m) {
+      col[i] = zmul(col[i], sc);
+      scol[r] = col[i];
+    }
+  }
+  if (lane == 0) {
+    sTau[j] = t;
+    sBeta[j] = bt;
+    sVn2[j] = v2;
+  }
+}
+
 // The same with each wave's column in registers, eight rows per lane
-// (m <= 512): per step a wave reads only the pivot column from LDS, and the
-// next pivot stores its updated column there for the step after.
+// (m <= 512).  The next pivot forms its reflector as soon as its own update
+// is done and leaves v (already scaled) in its LDS column with tau in the
+// panel's scalars; at a step the other waves then only form v^H a_c (one
+// two-value wave sum) and update their registers.
 static __device__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lm = m | 1;
   const bool own = wv < nbp;
   zd col[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int r = lane + 64 * i;
-    col[i] = (own && r < m) ? sA[r + (size_t)m * wv] : zmk(0.0, 0.0);
+    col[i] = (own && r < m) ? sA[r + (size_t)lm * wv] : zmk(0.0, 0.0);
   }
+  if (wv == 0) qf_reflector(col, sA, m, j0, 0, sTau, sBeta, sVn2);
+  __syncthreads();
   for (int j = 0; j < nbp; ++j) {
     const int dg = j0 + j;
-    if (own && wv >= j) {  // uniform per wave
-      const zd *xc = sA + (size_t)m * j;
-      zd xs[8];
+    if (own && wv > j) {  // uniform per wave
+      const zd *vc = sA + (size_t)lm * j;
+      zd vs[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int r = lane + 64 * i;
-        xs[i] = (r > dg && r < m) ? xc[r] : zmk(0.0, 0.0);
+        vs[i] = (r > dg && r < m) ? vc[r] : zmk(0.0, 0.0);
       }
-      double nx0 = 0.0, nx1 = 0.0, dr0 = 0.0, dr1 = 0.0, di0 = 0.0, di1 = 0.0;
+      double dr0 = 0.0, dr1 = 0.0, di0 = 0.0, di1 = 0.0;
 #pragma unroll
       for (int i = 0; i < 8; i += 2) {
-        nx0 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
-        nx1 += xs[i + 1].x * xs[i + 1].x + xs[i + 1].y * xs[i + 1].y;
-        dr0 += xs[i].x * col[i].x + xs[i].y * col[i].y;  // conj(x) y
-        dr1 += xs[i + 1].x * col[i + 1].x + xs[i + 1].y * col[i + 1].y;
-        di0 += xs[i].x * col[i].y - xs[i].y * col[i].x;
-        di1 += xs[i + 1].x * col[i + 1].y - xs[i + 1].y * col[i + 1].x;
+        dr0 += vs[i].x * col[i].x + vs[i].y * col[i].y;  // conj(v) a
+        dr1 += vs[i + 1].x * col[i + 1].x + vs[i + 1].y * col[i + 1].y;
+        di0 += vs[i].x * col[i].y - vs[i].y * col[i].x;
+        di1 += vs[i + 1].x * col[i + 1].y - vs[i + 1].y * col[i + 1].x;
       }
-      double nx = nx0 + nx1, dr = dr0 + dr1, di = di0 + di1;
-      wave_sum3(nx, dr, di);
-      const zd alpha = xc[dg];
-      zd t, sc, bt;
-      double v2;
-      if (nx == 0.0 && alpha.y == 0.0) {
-        t = zmk(0.0, 0.0);
-        sc = zmk(0.0, 0.0);
-        bt = alpha;
-        v2 = 1.0;
-      } else {
-        const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + nx), alpha.x);
-        const double rb = 1.0 / beta;
-        t = zmk((beta - alpha.x) * rb, -alpha.y * rb);
-        const double er = alpha.x - beta, rd = 1.0 / (er * er + alpha.y * alpha.y);
-        sc = zmk(er * rd, -alpha.y * rd);
-        bt = zmk(beta, 0.0);
-        v2 = 1.0 + nx * rd;
+      double dr = dr0 + dr1, di = di0 + di1;
+      wave_sum2(dr, di);
+      zd adg = zmk(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (i == (dg >> 6)) adg = col[i];
+      const zd acd = rdlane_zq(adg, dg & 63);
+      // w_c = tau^* v^H a_c with v_dg = 1; a_c -= v w_c
+      const zd wc = zmul(zconj(sTau[j]), zadd(acd, zmk(dr, di)));
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int r = lane + 64 * i;
+        if (r == dg) col[i] = zsub(col[i], wc);
+        else if (r > dg && r < m) col[i] = zsub(col[i], zmul(vs[i], wc));
       }
-      if (wv == j) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int r = lane + 64 * i;
-          if (r > dg && r < m) col[i] = zmul(col[i], sc);
-        }
-        if (lane == 0) {
-          sTau[j] = t;
-          sBeta[j] = bt;
-          sVn2[j] = v2;
-        }
-      } else {
-        // a_{dg,c}: this wave's register of row dg (lane dg % 64, slot dg / 64)
-        zd adg = zmk(0.0, 0.0);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-          if (i == (dg >> 6)) adg = col[i];
-        const zd acd = zmk(__hiloint2double(__builtin_amdgcn_readlane(__double2hiint(adg.x), dg & 63),
-                                            __builtin_amdgcn_readlane(__double2loint(adg.x), dg & 63)),
-                           __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(adg.y), dg & 63),
-                                            __builtin_amdgcn_readlane(__double2loint(adg.y), dg & 63)));
-        const zd wc = zmul(zconj(t), zadd(acd, zmul(zconj(sc), zmk(dr, di))));
-        const zd f = zmul(sc, wc);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int r = lane + 64 * i;
-          if (r == dg) col[i] = zsub(col[i], wc);
-          else if (r > dg && r < m) col[i] = zsub(col[i], zmul(xs[i], f));
-        }
-        if (wv == j + 1) {  // the next pivot column, for every wave's next step
-#pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const int r = lane + 64 * i;
-            if (r > dg && r < m) sA[r + (size_t)m * wv] = col[i];
-          }
-        }
-      }
+      if (wv == j + 1) qf_reflector(col, sA + (size_t)lm * wv, m, dg + 1, j + 1, sTau, sBeta, sVn2);
     }
     __syncthreads();
   }
@@ -430,7 +478,7 @@ static __device__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, 
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int r = lane + 64 * i;
-      if (r < m) sA[r + (size_t)m * wv] = col[i];
+      if (r < m) sA[r + (size_t)lm * wv] = col[i];
     }
   }
   __syncthreads();
@@ -445,8 +493,9 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
   __shared__ int sIdx, sStop;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m = a.m, nb = a.nb;
+  const int lm = m | 1;  // LDS column stride: odd, so the 16 columns of an MFMA operand hit 16 bank groups
   zd *sA = qf_lds;
-  double *sPart = reinterpret_cast<double *>(qf_lds + (size_t)m * nb);
+  double *sPart = reinterpret_cast<double *>(qf_lds + (size_t)lm * nb);
   if (tid == 0) {
     const unsigned long long t =
         __hip_atomic_fetch_add(&a.sync[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - a.ticket_base;
@@ -464,26 +513,31 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
   // the panel's columns, rows [0, m); columns past the matrix are zero
   for (int e = tid; e < m * nb; e += QF_THREADS) {
     const int r = e % m, c = e / m;
-    sA[e] = c < nbp ? a.A[r + (size_t)a.lda * (j0 + c)] : zmk(0.0, 0.0);
+    sA[r + lm * c] = c < nbp ? a.A[r + (size_t)a.lda * (j0 + c)] : zmk(0.0, 0.0);
   }
   __syncthreads();
   // 1. the reflectors of the panels to the left
   for (int p = 0; p < b; ++p) {
-    if (!qf_wait(a, p, &sStop)) return;
-    qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 1);
+    if (!qf_wait(a, 2 + p, &sStop)) return;
+    if (!qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 1, QF_TFLAG + p, &sStop)) return;
   }
   // 2. factor the panel, one wave per column
   if (m <= 512)
     qf_factor_regs(sA, m, j0, nbp, sTau, sBeta, sVn2);
   else
     qf_factor_lds(sA, m, j0, nbp, sTau, sBeta, sVn2);
-  // 3. publish V_b (0 above row j0 + c, 1 on it, the reflector below it);
-  // these stores drain while T is formed
+  // 3. publish V_b (0 above row j0 + c, 1 on it, the reflector below it)
+  // and raise its flag: the next panels' first products V_b^H A overlap the
+  // forming of T_b
   for (int e = tid; e < (m - j0) * nbp; e += QF_THREADS) {
     const int r = j0 + e % (m - j0), c = e / (m - j0);
-    const zd v = (r < j0 + c) ? zmk(0.0, 0.0) : (r == j0 + c ? zmk(1.0, 0.0) : sA[r + m * c]);
+    const zd v = (r < j0 + c) ? zmk(0.0, 0.0) : (r == j0 + c ? zmk(1.0, 0.0) : sA[r + lm * c]);
     qf_st_z(a.V + r + (size_t)m * (j0 + c), v);
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+  __syncthreads();
+  if (tid == 0) qf_st(&a.sync[2 + b], a.epoch);
   // T of zlarft from the Gram matrix Z = V^H V (rows [j0, m))
   const int npart = a.npart;
   if (wv < npart) {
@@ -492,7 +546,7 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
     for (int r0 = j0 + 4 * wv; r0 < m; r0 += 4 * npart) {
       const int r = r0 + lk;
       zd v = zmk(0.0, 0.0);
-      if (r < m && li < nbp) v = (r < j0 + li) ? zmk(0.0, 0.0) : (r == j0 + li ? zmk(1.0, 0.0) : sA[r + m * li]);
+      if (r < m && li < nbp) v = (r < j0 + li) ? zmk(0.0, 0.0) : (r == j0 + li ? zmk(1.0, 0.0) : sA[r + lm * li]);
       zr = mfma_f64(v.x, v.x, zr);
       zi = mfma_f64(v.x, v.y, zi);
       zr = mfma_f64(v.y, v.y, zr);
@@ -506,16 +560,17 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
     }
   }
   __syncthreads();
-  if (tid < 256) {  // Z(l, i) into sW2r / sW2i
+  if (tid < 256) {  // Z(l, i) into sW2r / sW2i, all loads first
     const int l = tid >> 4, i = tid & 15;
-    double zr = 0.0, zi = 0.0;
-    for (int w = 0; w < npart; ++w) {
-      const double *pr = sPart + (size_t)w * (QF_PART / 8);
-      zr += pr[l * 17 + i];
-      zi += pr[16 * 17 + l * 17 + i];
+    double zr[8], zi[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+      const double *pw = sPart + (size_t)w * (QF_PART / 8);
+      zr[w] = w < npart ? pw[l * 17 + i] : 0.0;
+      zi[w] = w < npart ? pw[16 * 17 + l * 17 + i] : 0.0;
     }
-    sW2r[l][i] = zr;
-    sW2i[l][i] = zi;
+    sW2r[l][i] = ((zr[0] + zr[1]) + (zr[2] + zr[3])) + ((zr[4] + zr[5]) + (zr[6] + zr[7]));
+    sW2i[l][i] = ((zi[0] + zi[1]) + (zi[2] + zi[3])) + ((zi[4] + zi[5]) + (zi[6] + zi[7]));
   }
   __syncthreads();
   // T^-1 = diag(1/tau) + striu(Z), so T = (I + N)^-1 diag(tau) with
@@ -570,7 +625,7 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
       if (own) sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
       __syncthreads();
     }
-    // T_b, then the flag once every store of V_b and T_b is complete
+    // T_b, then its flag once every store of it is complete
     if (own) qf_st_z(a.T + 256 * (size_t)b + i + 16 * k, (k < nbp && i < nbp) ? zmul(sWk[i + 16 * k], sTau[k]) : zmk(0.0, 0.0));
   }
   if (tid < nbp) {
@@ -581,29 +636,29 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
   __syncthreads();
-  if (tid == 0) qf_st(&a.sync[2 + b], a.epoch);
+  if (tid == 0) qf_st(&a.sync[QF_TFLAG + b], a.epoch);
   // 4. the panel's columns of A: R above the diagonal, beta on it, v below
   for (int e = tid; e < m * nbp; e += QF_THREADS) {
     const int r = e % m, c = e / m;
-    a.A[r + (size_t)a.lda * (j0 + c)] = (r == j0 + c) ? sBeta[c] : sA[e];
+    a.A[r + (size_t)a.lda * (j0 + c)] = (r == j0 + c) ? sBeta[c] : sA[r + lm * c];
   }
   if (!a.Q) return;
   __syncthreads();
   // Q(:, j0 + c) = H_0 ... H_b e_{j0 + c}
   for (int e = tid; e < m * nb; e += QF_THREADS) {
     const int r = e % m, c = e / m;
-    sA[e] = zmk((c < nbp && r == j0 + c) ? 1.0 : 0.0, 0.0);
+    sA[r + lm * c] = zmk((c < nbp && r == j0 + c) ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  for (int p = b; p >= 0; --p) qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 0);
+  for (int p = b; p >= 0; --p) qf_apply(a, sA, sPart, sW2r, sW2i, sTp, p, nbp, 0, -1, &sStop);
   for (int e = tid; e < m * nbp; e += QF_THREADS) {
     const int r = e % m, c = e / m;
-    a.Q[r + (size_t)m * (j0 + c)] = sA[e];
+    a.Q[r + (size_t)m * (j0 + c)] = sA[r + lm * c];
   }
 }
 
 extern "C" size_t pqmc_qr_flow_lds(int m, int nb, int *npart) {
-  const size_t base = sizeof(zd) * (size_t)m * nb;
+  const size_t base = sizeof(zd) * (size_t)(m | 1) * nb;
   const size_t budget = 160 * 1024 - 10 * 1024;  // less the kernel's static LDS
   const int np = (base + 8 * QF_PART <= budget) ? 8 : 4;
   if (npart) *npart = np;

@@ -185,6 +185,27 @@ PQ_DEV void wave_sum3(double &a, double &b, double &c) {
   c = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(c), 63), __builtin_amdgcn_readlane(__double2loint(c), 63));
 }
 
+// Two wave-wide sums with their DPP steps interleaved (the same operations
+// as wave_sum on each).
+PQ_DEV void wave_sum2(double &a, double &b) {
+#define PQ_STEP2(CTRL, ROWMASK)                   \
+  {                                               \
+    const double a_ = dpp_move<CTRL, ROWMASK>(a); \
+    const double b_ = dpp_move<CTRL, ROWMASK>(b); \
+    a += a_;                                      \
+    b += b_;                                      \
+  }
+  PQ_STEP2(0xB1, 0xF)
+  PQ_STEP2(0x4E, 0xF)
+  PQ_STEP2(0x141, 0xF)
+  PQ_STEP2(0x140, 0xF)
+  PQ_STEP2(0x142, 0xA)
+  PQ_STEP2(0x143, 0xC)
+#undef PQ_STEP2
+  a = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a), 63), __builtin_amdgcn_readlane(__double2loint(a), 63));
+  b = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(b), 63), __builtin_amdgcn_readlane(__double2loint(b), 63));
+}
+
 PQ_DEV double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
