@@ -240,6 +240,20 @@ int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P);
 /* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */
 int pqmc_mfma_selftest(double *max_abs_err);
 
+/* get_expKV_epsoc (src/mod_phonon_field.f90:572-604, selected through the
+ * get_expKV pointer for the EPSOC model, :472-496): the spin-up 2x2 bond
+ * exponential exp(-dtau H_b), H_b = [[0, h], [conj(h), 0]],
+ * h = K_coe Kmatrix(1,2) + V_coe x Vmatrix(1,2) times BC_phases(1,2) of a
+ * bond that crosses the boundary, in closed form as the reference evaluates
+ * it.  For pair i < n: family fam[i] in [0, nfam), field x[i], and (when
+ * cross is not null and cross[i] != 0) phase bcph[i]; out[8 i .. 8 i + 7] is
+ * E (inv = 0) or E^-1 (inv = 1), 2x2 complex column-major.  kcoe, vcoe,
+ * k12, v12: [nfam] complex (re, im pairs) as the reference's pf_type holds
+ * them.  Host arrays in and out; computed on the current device. */
+int pqmc_expkv_epsoc(double dtau, int32_t nfam, const double *kcoe, const double *vcoe, const double *k12,
+                     const double *v12, int64_t n, const int32_t *fam, const double *x, const int32_t *cross,
+                     const double *bcph, int32_t inv, double *out);
+
 /* Kernel timing hook for benchmarks: HIP events bracket every launch of
  * the named kernel class on the handle's stream when enabled; returns the
  * accumulated milliseconds and launch count. kind: 0 = family update,

@@ -535,6 +535,29 @@ void orc_get_expkv(const orc_model *m, int pf_id, int pla, double xv, int inv, d
   expkv(m, pf_id - 1, pla, xv, inv, (zc *)out);
 }
 
+/* get_expKV_epsoc of src/mod_phonon_field.f90:572-604, the EPSOC model's
+ * spin-up 2x2 bond: E = cosh(n) I2 + [[0, +-s sinh(n)/n],[conj(.), 0]],
+ * s = -delt (K_coe K(1,2) + V_coe x V(1,2)) (* BC_phases(1,2) when the bond
+ * crosses the boundary), n = delt sqrt(|K_coe|^2 + (|V_coe| x)^2); inv flips
+ * the sign of s.  out: 2x2 column-major.  ph: null when the bond does not
+ * cross. */
+void orc_expkv_epsoc(double dtau, const double *kcoe, const double *vcoe, const double *k12, const double *v12,
+                     const double *ph, double xv, int inv, double *out) {
+  const zc kc = CMPLX(kcoe[0], kcoe[1]), vc = CMPLX(vcoe[0], vcoe[1]);
+  const zc kk = CMPLX(k12[0], k12[1]), vv = CMPLX(v12[0], v12[1]);
+  double ak = cabs(kc), av = cabs(vc);
+  double norm = dtau * sqrt(ak * ak + (av * xv) * (av * xv));
+  zc sub = -dtau * (kc * kk + (vc * xv) * vv);
+  if (ph) sub = sub * CMPLX(ph[0], ph[1]);
+  const double ch = cosh(norm), sh = sinh(norm);
+  const zc e12 = inv ? (-sub) * sh / norm : sub * sh / norm;
+  zc *e = (zc *)out;
+  e[0] = ch;
+  e[1] = conj(e12);
+  e[2] = e12;
+  e[3] = ch;
+}
+
 static zc *EKV(const orc_model *m, zc *tab, int f, int b, int t /*1-based*/) {
   return tab + 16 * ((size_t)b + (size_t)m->npla * (f + 4 * (size_t)(t - 1)));
 }

@@ -61,6 +61,9 @@ void orc_set_trial(orc_model *m, const double *slater_c128);
 /* expKV tables (mod_phonon_field.f90:423-449, 605-628) */
 void orc_init_expkv(orc_model *m);
 void orc_get_expkv(const orc_model *m, int pf_id, int pla, double x, int inv, double *out_c128);
+/* get_expKV_epsoc (mod_phonon_field.f90:572-604): 2x2, complex pairs */
+void orc_expkv_epsoc(double dtau, const double *kcoe, const double *vcoe, const double *k12, const double *v12,
+                     const double *ph, double x, int inv, double *out_c128);
 
 /* G and B-string */
 void orc_init_g(orc_model *m, int start_time);

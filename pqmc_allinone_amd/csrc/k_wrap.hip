@@ -211,6 +211,35 @@ __global__ void k_ekv_build(model_t md, const double *x, const int *bfi /*[4*npl
   }
 }
 
+// The EPSOC model's 2x2 bond exponentials for n (family, field, phase)
+// triples: out[4 i .. 4 i + 3] is E (or E^-1) of triple i, column-major.
+__global__ void k_ekv_epsoc(double dtau, const zd *kcoe, const zd *vcoe, const zd *k12, const zd *v12, int nfam,
+                            long n, const int *fam, const double *x, const int *cross, const zd *ph, int inv,
+                            zd *out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int f = fam[i];
+    zd e[4];
+    if (f < 0 || f >= nfam) {  // refused on the host; never read outside the tables
+      e[0] = e[1] = e[2] = e[3] = zmk(0.0, 0.0);
+    } else {
+      expkv_epsoc(dtau, kcoe[f], vcoe[f], k12[f], v12[f], x[i], (ph && cross[i]) ? &ph[i] : nullptr, inv != 0, e);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[4 * i + q] = e[q];
+  }
+}
+
+extern "C" hipError_t pqmc_launch_ekv_epsoc(double dtau, const zd *kcoe, const zd *vcoe, const zd *k12,
+                                            const zd *v12, int nfam, long n, const int *fam, const double *x,
+                                            const int *cross, const zd *ph, int inv, zd *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_ekv_epsoc, dim3((unsigned)blocks), dim3(256), 0, s, dtau, kcoe, vcoe, k12, v12, nfam, n, fam,
+                     x, cross, ph, inv, out);
+  return hipGetLastError();
+}
+
 extern "C" hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, const int *bfi,
                                             const int *cross, const zd *ph, ekv_t *ekv, int t_lo,
                                             int t_hi, hipStream_t s) {

@@ -32,6 +32,9 @@ struct prop_args {
 extern "C" {
 hipError_t pqmc_launch_flush_wrap(const fw_args *a, hipStream_t s);
 hipError_t pqmc_launch_propagate(const prop_args *a, hipStream_t s);
+hipError_t pqmc_launch_ekv_epsoc(double dtau, const zd *kcoe, const zd *vcoe, const zd *k12, const zd *v12, int nfam,
+                                 long n, const int *fam, const double *x, const int *cross, const zd *ph, int inv,
+                                 zd *out, hipStream_t s);
 hipError_t pqmc_launch_ekv_build(const model_t *md, const double *x, const int *bfi, const int *cross,
                                  const zd *ph, ekv_t *ekv, int t_lo, int t_hi, hipStream_t s);
 hipError_t pqmc_zgemm(int opA, int opB, int M, int N, int K, zd alpha, const zd *A, int lda, const zd *B, int ldb,
@@ -1599,6 +1602,67 @@ extern "C" void pqmc_ran2_fill(pqmc_ran2 *s, int64_t n, float *out) {
     out[q] = amm * (float)((v < 0) ? v : ~v);
   }
   s->i = (int32_t)I; s->j = (int32_t)J; s->k = (int32_t)K; s->m = (int32_t)M; s->n = (int32_t)N;
+}
+
+// get_expKV_epsoc (src/mod_phonon_field.f90:572-604) for n (bond, slice)
+// pairs, on the current device; host arrays in and out.
+extern "C" int pqmc_expkv_epsoc(double dtau, int32_t nfam, const double *kcoe, const double *vcoe, const double *k12,
+                                const double *v12, int64_t n, const int32_t *fam, const double *x,
+                                const int32_t *cross, const double *bcph, int32_t inv, double *out) {
+  if (n < 0 || nfam <= 0 || !(dtau > 0.0)) return fail(PQMC_EINVAL, "expkv_epsoc: n >= 0, nfam > 0 and dtau > 0");
+  if (n == 0) return PQMC_OK;
+  if (!kcoe || !vcoe || !k12 || !v12 || !fam || !x || !out || (!bcph && cross))
+    return fail(PQMC_EINVAL, "expkv_epsoc: null argument");
+  for (int64_t i = 0; i < n; ++i)
+    if (fam[i] < 0 || fam[i] >= nfam) return fail(PQMC_EINVAL, "expkv_epsoc: family index out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PQMC_ENODEV, "no GPU");
+  zd *d_c = nullptr, *d_ph = nullptr, *d_out = nullptr;
+  int *d_fam = nullptr, *d_cross = nullptr;
+  double *d_x = nullptr;
+  hipStream_t st = nullptr;
+  int rc = PQMC_OK;
+  auto done = [&](int r) {
+    if (st) hipStreamDestroy(st);
+    hipFree(d_c);
+    hipFree(d_ph);
+    hipFree(d_out);
+    hipFree(d_fam);
+    hipFree(d_cross);
+    hipFree(d_x);
+    return r;
+  };
+#define EPCHK(expr)                                                                              \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return done(fail(PQMC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  EPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  EPCHK(dalloc(&d_c, 4 * (size_t)nfam));
+  EPCHK(dalloc(&d_fam, (size_t)n));
+  EPCHK(dalloc(&d_x, (size_t)n));
+  EPCHK(dalloc(&d_out, 4 * (size_t)n));
+  if (cross) {
+    EPCHK(dalloc(&d_cross, (size_t)n));
+    EPCHK(dalloc(&d_ph, (size_t)n));
+  }
+  const size_t zb = sizeof(zd) * nfam;
+  EPCHK(hipMemcpyAsync(d_c, kcoe, zb, hipMemcpyHostToDevice, st));
+  EPCHK(hipMemcpyAsync(d_c + nfam, vcoe, zb, hipMemcpyHostToDevice, st));
+  EPCHK(hipMemcpyAsync(d_c + 2 * nfam, k12, zb, hipMemcpyHostToDevice, st));
+  EPCHK(hipMemcpyAsync(d_c + 3 * nfam, v12, zb, hipMemcpyHostToDevice, st));
+  EPCHK(hipMemcpyAsync(d_fam, fam, sizeof(int) * n, hipMemcpyHostToDevice, st));
+  EPCHK(hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  if (cross) {
+    EPCHK(hipMemcpyAsync(d_cross, cross, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    EPCHK(hipMemcpyAsync(d_ph, bcph, sizeof(zd) * n, hipMemcpyHostToDevice, st));
+  }
+  EPCHK(pqmc_launch_ekv_epsoc(dtau, d_c, d_c + nfam, d_c + 2 * nfam, d_c + 3 * nfam, nfam, (long)n, d_fam, d_x,
+                              d_cross, d_ph, inv, d_out, st));
+  EPCHK(hipMemcpyAsync(out, d_out, sizeof(zd) * 4 * n, hipMemcpyDeviceToHost, st));
+  EPCHK(hipStreamSynchronize(st));
+#undef EPCHK
+  return done(rc);
 }
 
 extern "C" int pqmc_mfma_selftest(double *max_abs_err) {

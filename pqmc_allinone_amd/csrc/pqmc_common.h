@@ -86,6 +86,29 @@ PQ_DEV void expkv_compact(const model_t &md, int f, double x, const zd *ph, ekv_
   }
 }
 
+// get_expKV_epsoc (src/mod_phonon_field.f90:572-604): the EPSOC model's
+// spin-up 2x2 bond, E = cosh(n) I2 + [[0, e12],[conj(e12), 0]] with
+// e12 = +-s sinh(n)/n, s = -dtau (K_coe K(1,2) + V_coe x V(1,2)) times the
+// bond's BC_phases(1,2) when it crosses the boundary (ph, else null),
+// n = dtau sqrt(|K_coe|^2 + (|V_coe| x)^2); inv flips the sign of s.
+// out: 2x2 column-major.
+PQ_DEV void expkv_epsoc(double dtau, zd kc, zd vc, zd k12, zd v12, double x, const zd *ph, bool inv, zd out[4]) {
+  const double ak = hypot(kc.x, kc.y), av = hypot(vc.x, vc.y);
+  const double t = av * x;
+  const double norm = dtau * sqrt(ak * ak + t * t);
+  zd s = zadd(zmul(kc, k12), zmul(zscale(vc, x), v12));
+  s = zscale(s, -dtau);
+  if (ph) s = zmul(s, *ph);
+  const double ch = cosh(norm), sh = sinh(norm);
+  if (inv) s = zneg(s);
+  const zd t2 = zscale(s, sh);
+  const zd e12 = zmk(t2.x / norm, t2.y / norm);
+  out[0] = zmk(ch, 0.0);
+  out[1] = zconj(e12);
+  out[2] = e12;
+  out[3] = zmk(ch, 0.0);
+}
+
 // y = E v for a 4-vector (inv flips the sign of the off-diagonal blocks).
 PQ_DEV void ekv_apply(const ekv_t &e, bool inv, const zd v[4], zd y[4]) {
   zd m00 = e.m[0], m10 = e.m[1], m01 = e.m[2], m11 = e.m[3];

@@ -20,12 +20,21 @@
 module pqmc_dropin
   use iso_c_binding
   use pqmc_hip
+  ! The bond-family type.  In a reference build (-DPQMC_REFERENCE_TYPES) it
+  ! is the reference's own pf_type (src/input.f90:121-137), so the caller's
+  ! ppf_list passes straight through; the drop-in reads only its id.  The
+  ! stand-alone driver uses the minimal type below instead.
+#ifdef PQMC_REFERENCE_TYPES
+  use input, only: pf_type
+#endif
   implicit none
 
+#ifndef PQMC_REFERENCE_TYPES
   type :: pf_type
     integer :: id = 0
     logical :: V_exist = .true.
   end type pf_type
+#endif
 
   type(c_ptr), save :: chain = c_null_ptr
   type(pqmc_ran2), save :: rng_state

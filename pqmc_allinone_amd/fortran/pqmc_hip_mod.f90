@@ -186,6 +186,60 @@ module pqmc_hip
       complex(c_double_complex), intent(out) :: grad(*)
     end function pqmc_gradient
 
+    ! the same pass with the accumulator in the library's fixed device array
+    ! (no device allocation; at most 1,048,576 fields)
+    integer(c_int) function pqmc_gradient_noalloc(h, grad) bind(C, name='pqmc_gradient_noalloc')
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: h
+      complex(c_double_complex), intent(out) :: grad(*)
+    end function pqmc_gradient_noalloc
+
+    ! B-string products between QR checkpoints: 0 FP64, 1 FP32x3, 2 BF16x3
+    integer(c_int) function pqmc_set_precision(h, mode) bind(C, name='pqmc_set_precision')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function pqmc_set_precision
+
+    ! stabilization QR: 0 one launch (default), 1 panel by panel
+    integer(c_int) function pqmc_set_qr_mode(h, mode) bind(C, name='pqmc_set_qr_mode')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode
+    end function pqmc_set_qr_mode
+
+    ! the latest sweep's largest G and Re ln_cw drifts
+    integer(c_int) function pqmc_get_drift(h, out2) bind(C, name='pqmc_get_drift')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out2(2)
+    end function pqmc_get_drift
+
+    ! every stabilization's (G, Re ln_cw) drift pair of the latest sweep
+    integer(c_int) function pqmc_get_drift_history(h, out, cap, n) bind(C, name='pqmc_get_drift_history')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: n
+    end function pqmc_get_drift_history
+
+    ! get_expKV_epsoc (src/mod_phonon_field.f90:572-604) for n (bond, slice)
+    ! pairs: out(2, 2, n) holds E or (inv /= 0) E^-1; bcph may be c_null_ptr
+    ! with cross when no bond crosses
+    integer(c_int) function pqmc_expkv_epsoc(dtau, nfam, kcoe, vcoe, k12, v12, n, fam, x, cross, bcph, inv, out) &
+        bind(C, name='pqmc_expkv_epsoc')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr
+      real(c_double), value :: dtau
+      integer(c_int32_t), value :: nfam, inv
+      complex(c_double_complex), intent(in) :: kcoe(*), vcoe(*), k12(*), v12(*)
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), intent(in) :: fam(*)
+      real(c_double), intent(in) :: x(*)
+      type(c_ptr), value :: cross, bcph
+      complex(c_double_complex), intent(out) :: out(*)
+    end function pqmc_expkv_epsoc
+
     integer(c_int) function pqmc_get_accepts(h, acc, n) bind(C, name='pqmc_get_accepts')
       import :: c_int, c_ptr, c_int8_t, c_int64_t
       type(c_ptr), value :: h

@@ -57,6 +57,33 @@ def observable_layout(ncell: int):
     return out
 
 
+def expkv_epsoc(dtau: float, kcoe, vcoe, k12, v12, fam, x, bcph=None, cross=None, inv: bool = False):
+    """The EPSOC model's 2x2 bond exponentials (get_expKV_epsoc,
+    src/mod_phonon_field.f90:572-604) on the device: kcoe, vcoe, k12, v12
+    per family (complex); fam, x (and the boundary phase bcph where cross is
+    set) per (bond, slice) pair.  Returns (n, 2, 2) complex."""
+    L = _lib.lib()
+    kc, vc, kk, vv = (np.ascontiguousarray(np.asarray(a, np.complex128)) for a in (kcoe, vcoe, k12, v12))
+    fam = np.ascontiguousarray(fam, np.int32)
+    x = np.ascontiguousarray(x, np.float64)
+    n = fam.size
+    if x.size != n:
+        raise ValueError("fam and x need one entry per pair")
+    cr = ph = None
+    if cross is not None:
+        cr = np.ascontiguousarray(cross, np.int32)
+        ph = np.ascontiguousarray(np.asarray(bcph, np.complex128))
+        if cr.size != n or ph.size != n:
+            raise ValueError("cross and bcph need one entry per pair")
+    out = np.zeros((n, 2, 2), np.complex128)
+    work = np.zeros(4 * n, np.complex128)
+    _lib.check(L.pqmc_expkv_epsoc(float(dtau), int(kc.size), _lib.ptr(kc), _lib.ptr(vc), _lib.ptr(kk), _lib.ptr(vv),
+                                  int(n), _lib.ptr(fam), _lib.ptr(x), _lib.ptr(cr) if cr is not None else None,
+                                  _lib.ptr(ph) if ph is not None else None, int(bool(inv)), _lib.ptr(work)))
+    out[:] = work.reshape(n, 2, 2).transpose(0, 2, 1)  # column-major 2x2 per pair
+    return out
+
+
 class Ran2:
     """ran2 of src/mod_ranstate.f90:171-191, seeded as init_rng_by_seed (:267-283)."""
 

@@ -51,6 +51,7 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p, C.c_void_p]
         L.orc_get_qstring.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_get_expkv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p]
+        L.orc_expkv_epsoc.argtypes = [C.c_double] + [C.c_void_p] * 5 + [C.c_double, C.c_int, C.c_void_p]
         L.orc_init_g.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_ft.argtypes = [C.c_void_p, C.c_int]
         L.orc_left_evolve.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -298,3 +299,14 @@ def zdet(a):
     out = np.zeros(2)
     lib().orc_zdet(a.shape[0], P(a), P(out))
     return complex(out[0], out[1])
+
+
+def expkv_epsoc(dtau, kcoe, vcoe, k12, v12, x, ph=None, inv=False):
+    """The oracle's get_expKV_epsoc (src/mod_phonon_field.f90:572-604): one
+    2x2 bond exponential; ph is the BC_phases(1,2) of a crossing bond."""
+    a = [np.array([complex(v)], np.complex128) for v in (kcoe, vcoe, k12, v12)]
+    p = np.array([complex(ph)], np.complex128) if ph is not None else None
+    out = np.zeros(4, np.complex128)
+    lib().orc_expkv_epsoc(float(dtau), *[P(v) for v in a], P(p) if p is not None else None, float(x), int(bool(inv)),
+                          P(out))
+    return out.reshape(2, 2).T  # column-major

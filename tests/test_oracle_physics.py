@@ -10,7 +10,10 @@ Pins, per function of the oracle:
   scratch with the new field;
 * take_measurement: Wick contractions vs Fock-space many-body expectation
   values, k-space companions vs an explicit Fourier sum;
-* a full sweep vs the dense-product G at the sweep's end.
+* a full sweep vs the dense-product G at the sweep's end;
+* get_g_scratch_T0's drift check: the returned error is max|G_fast - G| of the
+  oracle's own matrices before and after, and err_fast_max is the running
+  maximum of it and of |Re(ln_cw - ln_cw_fast)| from 1e-6 on.
 """
 import numpy as np
 import pytest
@@ -54,6 +57,27 @@ def test_expkv_closed_form_matches_expm(bc):
     e0 = orc.expkv(1, 0, 0.0, 0)
     assert abs(e0[0, 0] - np.cosh(0.1)) < 1e-15
     assert abs(abs(e0[0, 2]) - np.sinh(0.1)) < 1e-15
+
+
+@pytest.mark.parametrize("kc,vc,ph", [(-1.0, 0.64, None), (-1.0, 0.64, np.exp(1j * np.pi * 0.3)),
+                                       (-0.7 * np.exp(0.4j), 1.3 * np.exp(0.4j), np.exp(-1j * np.pi * 0.5))])
+def test_expkv_epsoc_matches_expm(kc, vc, ph):
+    """get_expKV_epsoc's closed form against scipy's expm of -dtau H_b for the
+    EPSOC model's bond (input/model/EPSOC.nml: Kmatrix(1,2) = 1,
+    Vmatrix(1,2) = -i), H_b = [[0, h], [conj(h), 0]],
+    h = (K_coe Kmatrix(1,2) + V_coe x Vmatrix(1,2)) * phase; the inverse
+    against expm(+dtau H_b).  The closed form's n = dtau |h| holds for these
+    matrices and for coefficients with a common phase."""
+    import scipy.linalg as sla
+    dtau, k12, v12 = 0.1, 1.0, -1j
+    for x in (0.0, 0.37, -2.3, 5.0):
+        h = (kc * k12 + vc * x * v12) * (ph if ph is not None else 1.0)
+        H = np.array([[0, h], [np.conj(h), 0]])
+        e = O.expkv_epsoc(dtau, kc, vc, k12, v12, x, ph)
+        ei = O.expkv_epsoc(dtau, kc, vc, k12, v12, x, ph, inv=True)
+        assert np.abs(e - sla.expm(-dtau * H)).max() < 1e-14, x
+        assert np.abs(ei - sla.expm(dtau * H)).max() < 1e-14, x
+        assert np.abs(e @ ei - np.eye(2)).max() < 1e-14, x
 
 
 def test_model_tables_match_host_builder():
@@ -438,3 +462,41 @@ def test_factorizations_follow_lapack():
         assert np.allclose(np.log(dl), np.log(dlap), rtol=0, atol=1e-13)
         nneg += int((dl.real < 0).sum())
     assert nneg > 0  # some diagonal is negative, so the branch is exercised
+
+
+def test_drift_check_is_the_running_maximum():
+    """get_g_scratch_T0 (src/mod_evolution.f90:179-193): after a backward
+    stretch of wraps and local updates, the value returned is max|G_fast - G|
+    recomputed here from the matrices before and after the call, and
+    err_fast_max becomes max(err_fast_max, that, |Re(ln_cw - ln_cw_fast)|),
+    where ln_cw_fast is the weight the accepts carried up to the call."""
+    L, Lt, ng = 4, 40, 10
+    m, x, sl, orc = setup(L=L, Lt=Lt, ngroup=ng, ne=16, seed=9)
+    rng = O.Ran2Stream(21)
+    _, _, efm = orc.counters()
+    assert efm == 1e-6  # src/input.f90:53
+    time = Lt
+    for block in range(3):
+        for _ in range(ng):
+            for f in (4, 3, 2, 1):
+                orc.wrap_g(time, f, False)
+                orc.update_phonon_field(f, time, rng.draws(2 * m.npla))
+            time -= 1
+        if block == 1:  # a planted drift above the initial 1e-6 moves the maximum
+            gp = orc.g.copy()
+            gp[3, 5] += 3e-6
+            orc.g = gp
+        g0, l0 = orc.g.copy(), orc.ln_cw
+        _, _, efm0 = orc.counters()
+        e = orc.get_g_scratch(time + 1, False)
+        eg = float(np.abs(g0 - orc.g).max())
+        el = abs((orc.ln_cw - l0).real)
+        _, _, efm1 = orc.counters()
+        # cabs and numpy's abs may differ in the last bit
+        assert abs(e - eg) <= 1e-12 * eg, (block, e, eg)
+        assert 0 < eg < (4e-6 if block == 1 else 1e-8)
+        assert abs(efm1 - max(efm0, e, el)) <= 1e-12 * efm1, (block, efm0, e, el, efm1)
+        print(f"block {block}: G drift {eg:.2e}, Re ln_cw drift {el:.2e}, err_fast_max {efm1:.2e}")
+    pa, na, efm = orc.counters()
+    assert pa + na > 0  # the stretch accepted bonds, so ln_cw_fast moved
+    assert 2.9e-6 < efm < 3.1e-6  # kept from block 1 through block 2
