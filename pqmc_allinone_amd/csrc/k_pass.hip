@@ -350,6 +350,7 @@ struct ps_leader_lds {
   int npub, done;    // wave 0 -> wave 1: accepts handed over, block decided
   int nfold, bdone;  // wave 1 -> waves 2-5: accepts folded into A and T^-1, block folded
   int tiles;         // waves 2-5 that have stored their part of the fold tiles
+  int wdone;         // wave 6 -> wave 0: accepts whose update of W off its diagonal 4x4 blocks is done
   int nacc, abort;
 };
 
@@ -495,6 +496,48 @@ static __device__ void ps_pq_accept(ps_leader_lds &L, int m, int t2, const zd *T
   }
 }
 
+// Wave 6: W += u_m v_m on the later rows and columns of accept m, except the
+// diagonal 4 x 4 blocks of the later bonds (the deciding wave updates those),
+// on FP64 MFMA: per 16 x 16 tile of W that meets them, D += U V with U the
+// u_m rows (zero above row o) and V the v_m columns (zero left of column o),
+// complex as four real products of K = 4.  Lane l supplies U[l % 16][l / 16]
+// and V[l / 16][l % 16] and holds rows l / 16 + 4 i of column l % 16.  Every
+// entry of W has one writer, in accept order, so the result does not depend
+// on how the two waves interleave.
+static __device__ void ps_w_offdiag(ps_leader_lds &L, zd *W, int m, int nsk) {
+  const int lane = threadIdx.x & 63;
+  const int o = 4 * (L.li[m] + 1);
+  if (o >= nsk) return;
+  const int li = lane & 15, lk = lane >> 4;
+  const int t0 = o >> 4, t1 = (nsk - 1) >> 4;
+  for (int rt = t0; rt <= t1; ++rt) {
+    const int ra = 16 * rt + li;
+    const zd ua = zsel(ra >= o && ra < nsk, L.sU[m][ra * 4 + lk]);
+    for (int ct = t0; ct <= t1; ++ct) {
+      const int cb = 16 * ct + li;
+      const zd vb = zsel(cb >= o && cb < nsk, L.sV[m][lk * PS_BS + cb]);
+      double4_t dre, dim;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const zd t = W[(16 * rt + lk + 4 * i) * PS_BS + cb];
+        dre[i] = t.x;
+        dim[i] = t.y;
+      }
+      dre = mfma_f64(ua.x, vb.x, dre);
+      dim = mfma_f64(ua.x, vb.y, dim);
+      dre = mfma_f64(-ua.y, vb.y, dre);
+      dim = mfma_f64(ua.y, vb.x, dim);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * rt + lk + 4 * i;
+        if (row >= o && row < nsk && cb >= o && cb < nsk && (row >> 2) != (cb >> 2))
+          W[row * PS_BS + cb] = zmk(dre[i], dim[i]);
+      }
+    }
+  }
+  wave_sync();
+}
+
 static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const model_t &md = A.md;
@@ -515,6 +558,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     L.nfold = 0;
     L.bdone = 0;
     L.tiles = 0;
+    L.wdone = 0;
     L.nacc = ps_wait(A, ps_pub(A, -1, 0), ps_owners(A, 0), nullptr, 0) ? 1 : 0;
   }
   __syncthreads();
@@ -646,6 +690,9 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
           st_coh_ekv(&A.ekv[((size_t)(A.t - 1) * 4 + A.f) * npla + p0 + j], en);
         }
         wave_sync();
+        // u_m and v_m read W off its diagonal blocks: wave 6 has applied the
+        // earlier accepts there
+        if (m > 0 && L.wdone < m) ps_lds_progress(A, L, &L.wdone, &L.abort, m - 1);
         PROF_ACC(5);
         // u_m, v_m at the later sites of the block
         const int o = 4 * (j + 1), Ln = nsk - o;
@@ -662,38 +709,21 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         // hand accept m to wave 1
         if (lane == 0) __hip_atomic_store(&L.npub, m + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         PROF_ACC(6);
-        // W += u_m v_m on the later rows and columns, on FP64 MFMA: per 16 x 16
-        // tile of W that meets them, D += U V with U the u_m rows (zero above
-        // row o) and V the v_m columns (zero left of column o), complex as four
-        // real products of K = 4.  Lane l supplies U[l % 16][l / 16] and
-        // V[l / 16][l % 16] and holds rows l / 16 + 4 i of column l % 16.
-        if (Ln > 0) {
-          const int li = lane & 15, lk = lane >> 4;
-          const int t0 = o >> 4, t1 = (nsk - 1) >> 4;
-          for (int rt = t0; rt <= t1; ++rt) {
-            const int ra = 16 * rt + li;
-            const zd ua = zsel(ra >= o && ra < nsk, L.sU[m][ra * 4 + lk]);
-            for (int ct = t0; ct <= t1; ++ct) {
-              const int cb = 16 * ct + li;
-              const zd vb = zsel(cb >= o && cb < nsk, L.sV[m][lk * PS_BS + cb]);
-              double4_t dre, dim;
+        // W += u_m v_m on the diagonal 4 x 4 blocks of the later bonds, the
+        // only part of W the decision rounds read; wave 6 updates the rest
+        // (ps_w_offdiag) while the next round runs
+        for (int e = lane; e < Ln * 4; e += 64) {
+          const int row = o + (e >> 2), col = 4 * (row >> 2) + (e & 3);
+          zd uv[4], vv[4];
 #pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const zd t = W[(16 * rt + lk + 4 * i) * PS_BS + cb];
-                dre[i] = t.x;
-                dim[i] = t.y;
-              }
-              dre = mfma_f64(ua.x, vb.x, dre);
-              dim = mfma_f64(ua.x, vb.y, dim);
-              dre = mfma_f64(-ua.y, vb.y, dre);
-              dim = mfma_f64(ua.y, vb.x, dim);
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int row = 16 * rt + lk + 4 * i;
-                if (row >= o && row < nsk && cb >= o && cb < nsk) W[row * PS_BS + cb] = zmk(dre[i], dim[i]);
-              }
-            }
+          for (int c = 0; c < 4; ++c) {
+            uv[c] = L.sU[m][row * 4 + c];
+            vv[c] = L.sV[m][c * PS_BS + col];
           }
+          zd acc = W[row * PS_BS + col];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc = zadd(acc, zmul(uv[c], vv[c]));
+          W[row * PS_BS + col] = acc;
         }
         wave_sync();
         PROF_ACC(7);
@@ -748,6 +778,17 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         L.abort = 1;
       }
       PROF_ACC(5);
+    } else if (wave == 6) {
+      // ---- W off its diagonal blocks, accept by accept ----
+      int md_ = 0;
+      for (;;) {
+        const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
+        if (np <= md_) break;
+        for (; md_ < np; ++md_) {
+          ps_w_offdiag(L, L.buf[iW], md_, nsk);
+          if (lane == 0) __hip_atomic_store(&L.wdone, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
     } else if (has_next && wave == 7) {
       // ---- wave 7: the next block's proposals ----
       const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
@@ -820,6 +861,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       L.nfold = 0;
       L.bdone = 0;
       L.tiles = 0;
+      L.wdone = 0;
     }
     __syncthreads();
     // the window of block k+1 is T11; the old window's buffer becomes T11
