@@ -747,15 +747,25 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         for (; md_ < np; ++md_) {
           ps_fold_accept(L, md_, lane);
           if (lane == 0) __hip_atomic_store(&L.nfold, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          // the record's entries that accept md_ fixed: T^-1 rows 4 md_..4 md_+3
+          // left of the diagonal block, A columns 4 md_..4 md_+3 down to row
+          // 4 md_+3; later accepts leave them unchanged
+          ps_rec *R = A.rec + k;
+          const int m4 = 4 * md_;
+          for (int e = lane; e < 4 * m4 + 4 * (m4 + 4); e += 64) {
+            const int r = e < 4 * m4 ? m4 + e / m4 : (e - 4 * m4) >> 2;
+            const int c = e < 4 * m4 ? e % m4 : m4 + ((e - 4 * m4) & 3);
+            st_coh(&R->AT[r * PS_BS + c], L.sAT[r * PS_BS + c]);
+          }
         }
         PROF_ACC(1);
       }
       if (lane == 0) __hip_atomic_store(&L.bdone, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      // the record's data now; its flag after the barrier
-      const int a = L.nacc, n4 = 4 * a;
+      // the record's accepted indices now (its A and T^-1 went out accept by
+      // accept); its flag after the barrier
+      const int a = L.nacc;
       ps_rec *R = A.rec + k;
       if (lane < a) st_coh_i(&R->li[lane], L.li[lane]);
-      for (int e = lane; e < n4 * n4; e += 64) st_coh(&R->AT[e], L.sAT[(e / n4) * PS_BS + e % n4]);
       PROF_ACC(4);
     } else if (has_next && wave <= 5) {
       // ---- waves 2..5: the fold tiles after step k-1, then P and Q ----
@@ -1111,7 +1121,10 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     if (a > 0) {
       const ps_rec *R = A.rec + k;
       if (tid < a) Hh.li[tid] = R->li[tid];
-      for (int e = tid; e < n4 * n4; e += PS_THREADS) Hh.sAT[(e / n4) * PS_BS + e % n4] = R->AT[e];
+      for (int e = tid; e < n4 * n4; e += PS_THREADS) {
+        const int r = e / n4, c = e % n4;
+        Hh.sAT[r * PS_BS + c] = R->AT[r * PS_BS + c];
+      }
       __syncthreads();
       PROF_ACC(2);
       // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q] = A (T^-1 w[q]) as
