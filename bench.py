@@ -363,9 +363,9 @@ def kernel_classes(reps):
     """The path's kernel classes other than the family pass, on the 14x14,
     Lt=1000 chain, for counter collection: init_g(1), then `reps` each of
     get_g_scratch (B-string propagation, blocked Householder QR, G from
-    scratch), take_measurement, the standalone two-way wrap, the field
-    gradient's wrap (a gradient pass over the first 10 slices), and the
-    mixed-precision string products (FP32 split and BF16x3)."""
+    scratch), take_measurement and the standalone two-way wrap, then
+    `reps` stabilizations with the string products in each reduced
+    precision (FP32 split and BF16x3)."""
     from pqmc_allinone_amd.host import Chain
     from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
     L, Lt, ng, ne, seed = CONFIGS["c2"]
