@@ -10,10 +10,17 @@
 #define PS_BS (4 * PS_NB)   // sites per block
 #define PS_WAIT_TICKS 50000000ull  // 0.5 s of the 100 MHz wall clock
 
-// What a helper needs of a decided block: the a accepted bonds and the
-// 4a x 4a factors A (block upper) and T^-1 (block lower, identity diagonal
-// blocks) of B = A T^-1, with
-//     G_after[:, q] = G_before[:, q] + G_before[:, S_A] B w[q],  w[q] = (I - G_before)[S_A, q].
+// What a helper needs of a decided block: the a accepted bonds and, for the
+// rank-4 updates G <- G + u_m v_m of accepts m = 0..a-1 (u_m = G[:, S_m] X_m,
+// v_m = (I - G)[S_m, :], G current before accept m), the 4 x 4 blocks
+//     X_m                           on the diagonal block (m, m),
+//     T[m][m'] = u_m'[S_m]          below it (m' < m),
+//     X_m' V[m'][m] = X_m' v_m'[S_m] above it (m' < m),
+// row-major with stride PS_BS.  With them a helper forms the folded update
+// of its own columns q from the pivot columns at the block's start:
+//     v_m(q) = w_m(q) - sum_{m'<m} T[m][m'] v_m'(q),    w_m(q) = (I - G)[S_m, q],
+//     y_m = X_m v_m(q) + sum_{m'>m} (X_m V[m][m']) y_m'   (m = a-1 .. 0),
+//     G_after[:, q] = G[:, q] + sum_m G[:, S_m] y_m.
 // The accept count itself travels in the block's flag word.
 // Helpers read a record with plain, L2-cached loads after its flag, so no
 // 128-byte line of one record may also hold part of another: a line cached
@@ -23,7 +30,7 @@
 struct __align__(128) ps_rec {
   int li[PS_NB];          // block-local indices of the accepted bonds, ascending
   int pad[32 - PS_NB];
-  zd AT[PS_BS * PS_BS];   // A on and above the diagonal blocks, T^-1 below; row-major, stride PS_BS
+  zd AT[PS_BS * PS_BS];   // X on the diagonal blocks, X V above, T below; row-major, stride PS_BS
 };
 static_assert(sizeof(ps_rec) % 128 == 0, "a decision record must be a whole number of 128-byte lines");
 static_assert(offsetof(ps_rec, AT) == 128, "the record header must fill exactly one 128-byte line");
@@ -39,13 +46,15 @@ struct ps_args {
   ekv_t *ekv;         // entry ((t-1)*4 + f)*npla + b
   const float *u;     // 2*npla draws: (ran_sysm source, rands) per bond
   zd *G;              // Ns x Ns, column-major, in/out
-  zd *slot;           // [nblk + 1][2][PS_BS][lds] columns of blocks k+1, k+2 after step k = -1..nblk-1
+  zd *slot;           // [nblk + 1][2][PS_BS][lds] columns of block k+1 after step k = -1..nblk-1 (the
+                      // second half of each step's slot is not used)
   int lds;            // column stride of the slot: Ns rounded up to whole 128-byte lines
   zd *tslot;          // [nblk + 1][2][PS_BS][2 PS_BS] the same columns at the rows of blocks k+1, k+2
   ps_rec *rec;        // [nblk]
   unsigned long long *sync;  // [1] done, [2] abort, [8 + 2(k + 1) + mb] owners of block k+1+mb that have
-                             // published after step k (k = -1..nblk-1), [10 + 2 nblk + k] block k
-                             // decided: accepts + 1
+                             // published their tile-slot columns after step k (k = -1..nblk-1),
+                             // [10 + 2 nblk + k] block k decided: accepts + 1, [10 + 3 nblk + k + 1]
+                             // owners of block k+1 that have published their slot columns after step k
   uint8_t *acc;       // [npla]
   double *lncw_acc;
   unsigned long long *stats;  // [0] accepted, [1] rejected, [2] wait timeouts
@@ -62,7 +71,7 @@ static inline size_t pqmc_pass_slot_elems(int Ns, int nblk) {
 }
 static inline size_t pqmc_pass_tslot_elems(int nblk) { return (size_t)(nblk + 1) * 2 * PS_BS * 2 * PS_BS; }
 // words of the sync array for nblk decision blocks
-static inline size_t pqmc_pass_sync_words(int nblk) { return 16 + 3 * (size_t)nblk; }
+static inline size_t pqmc_pass_sync_words(int nblk) { return 16 + 4 * (size_t)nblk; }
 
 extern "C" {
 size_t pqmc_pass_lds(int Ns, int bph);

@@ -8,33 +8,34 @@
 //     G <- G + u_i v_i,   u_i = G[:, S_i] X_i,   v_i = (I - G)[S_i, :],
 //     X_i = Delta_i (I + G[S_i, S_i] Delta_i)^-1.
 // The bonds of one family act on disjoint sites, so the updates of the a
-// accepted bonds of any stretch of bonds fold into one update of rank 4a:
-//     G_after[:, q] = G[:, q] + G[:, S_A] B w[q],   w[q] = (I - G)[S_A, q],
-// where G is taken before the stretch and the 4a x 4a matrix B = A T^-1
-// collects the sequence:
-//     A[:, m] = (sel_m + sum_{m'<m} A[:, m'] v_{m'}[S_m]) X_m   (block upper),
-//     T[m][m'] = u_{m'}[S_m] for m' < m, identity blocks on the diagonal.
-// T^-1 is block lower with identity diagonal blocks, so A and T^-1 share one
-// 4a x 4a array: A in the upper part with the diagonal blocks, T^-1 in the
-// strictly lower part.  The bonds are cut into decision blocks of nb
-// consecutive bonds; one leader workgroup decides them and H helper
-// workgroups own the columns of G:
+// accepted bonds of any stretch of bonds fold into one update of the
+// stretch's start G:
+//     G_after[:, q] = G[:, q] + sum_m G[:, S_m] y_m(q),
+//     v_m(q) = w_m(q) - sum_{m'<m} T[m][m'] v_m'(q),   w_m(q) = (I - G)[S_m, q],
+//     y_m(q) = X_m v_m(q) + sum_{m'>m} (X_m V[m][m']) y_m'(q)   (m = a-1 .. 0),
+// with the 4 x 4 blocks T[m][m'] = u_m'[S_m] and V[m'][m] = v_m'[S_m]
+// (forward and backward block substitutions).  The bonds are cut into
+// decision blocks of nb consecutive bonds; one leader workgroup decides them
+// and H helper workgroups own the columns of G:
 //
 //  * Leader, wave 0: decides the bonds of block k from an LDS copy
 //    W = G[S_k, S_k] that it keeps current with the rank-4 updates (only the
-//    rows and columns of the block's later bonds change).
-//  * Leader, wave 1: extends A and T^-1 by each accept while wave 0 goes on,
-//    and publishes them with the accepted indices.
+//    rows and columns of the block's later bonds change); wave 6 takes the
+//    part of each update off W's diagonal 4 x 4 blocks.
+//  * Leader, wave 1: copies each accept's X_m, T and X V blocks (from wave
+//    0's u_m, v_m at the block's sites) to the block's record and publishes
+//    it as soon as the block is decided.
 //  * Leader, waves 2-5: fetch G[S_{k+1}, S_k], G[S_k, S_{k+1}] and
-//    G[S_{k+1}, S_{k+1}] after block k-1 and, accept by accept, the factors
-//    P = G[S_{k+1}, S_A] A and Q = T^-1 G[S_A, S_{k+1}]; after block k they
-//    fold it into the next window,  G[S_{k+1}, S_{k+1}] -= P Q.  The helpers
+//    G[S_{k+1}, S_{k+1}] as they were after block k-1 into MFMA
+//    accumulators and apply block k's accepts to them one by one, so that
+//    the window of block k+1 is ready when block k is decided.  The helpers
 //    therefore lag by one block.  Wave 7 prepares the next block's proposals.
 //  * Helper h keeps the columns of bph bonds in LDS for the whole pass and
-//    applies block k to them in one step, G[:, q] += G[:, S_A] (B w[q]) with
-//    B = A T^-1 formed locally, and the pivot columns G[:, S_A] read from a
-//    slot that the owners of block k's columns filled after step k-1.
-//
+//    applies block k to them in one step from the record, with the pivot
+//    columns G[:, S_A] from a slot that the owners of block k's columns
+//    filled after step k-1 (with one bond per helper, loaded into registers
+//    while the leader decides the block).
+
 // Accept test: the reference rejects when prob < u with prob =
 // min(1, exp(min(10, w))), w = dS + ncopy*(1+TR)*log|det(I + G_SS Delta)|
 // (cal_prob, :475-500).  As log u < 0 this is exactly w < log u, i.e.
@@ -112,7 +113,7 @@ static __device__ __forceinline__ void ps_st(unsigned long long *p, unsigned lon
 // Single-lane wait until *p >= v (and *q >= w when q is set); returns the
 // last value read from p, or 0 when the wait failed.  Bounded by wall-clock
 // time and by the abort word.
-static __device__ unsigned long long ps_wait(const ps_args &A, unsigned long long *p, unsigned long long v,
+static __device__ __forceinline__ unsigned long long ps_wait(const ps_args &A, unsigned long long *p, unsigned long long v,
                                              unsigned long long *q, unsigned long long w) {
   // one round trip per poll: the flag words are loaded together; the abort
   // word and the clock are looked at every 32 polls
@@ -146,6 +147,10 @@ static __device__ __forceinline__ unsigned long long *ps_pub(const ps_args &A, i
 }
 static __device__ __forceinline__ unsigned long long *ps_flag(const ps_args &A, int k) {
   return &A.sync[10 + 2 * A.nblk + k];
+}
+// count of the owners of block kstep+1 whose slot columns are out after step kstep
+static __device__ __forceinline__ unsigned long long *ps_spub(const ps_args &A, int kstep) {
+  return &A.sync[10 + 3 * A.nblk + kstep + 1];
 }
 // helpers that own at least one bond of block blk
 static __device__ __forceinline__ unsigned long long ps_owners(const ps_args &A, int blk) {
@@ -195,6 +200,13 @@ static __device__ __forceinline__ double row16_max(double v) {
   v = fmax(v, dpp_move<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
   v = fmax(v, dpp_move<0x141, 0xF>(v));  // row_half_mirror
   v = fmax(v, dpp_move<0x140, 0xF>(v));  // row_mirror
+  return v;
+}
+
+// sum over the four lanes of each quad, in every lane of the quad
+static __device__ __forceinline__ double quad_sum(double v) {
+  v += dpp_move<0xB1, 0xF>(v);  // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E, 0xF>(v);  // quad_perm [2,3,0,1]
   return v;
 }
 
@@ -253,7 +265,8 @@ static __device__ __forceinline__ zd minor3(const zd *M, int ri, int cj) {
     if (A.prof && (threadIdx.x & 63) == 0)                                                \
       for (int _k = 0; _k < 8; ++_k) A.prof[(row) * 8 + _k] += _pa[_k];                   \
   } while (0)
-// rows: 0 leader wave 0, 1..H helpers, H+1 leader wave 1, H+2 leader wave 2
+// rows: 0 leader wave 0, 1..H helpers, H+1 leader wave 1, H+2 leader wave 2,
+// H+3 leader wave 6, H+4 leader wave 7
 #define PROF_FLUSH()                                                                      \
   do {                                                                                    \
     if (threadIdx.x == 0)                                                                 \
@@ -262,12 +275,16 @@ static __device__ __forceinline__ zd minor3(const zd *M, int ri, int cj) {
       PROF_FLUSH_ROW(A.H + 1);                                                            \
     else if (blockIdx.x == 0 && threadIdx.x == 128)                                       \
       PROF_FLUSH_ROW(A.H + 2);                                                            \
+    else if (blockIdx.x == 0 && threadIdx.x == 384)                                       \
+      PROF_FLUSH_ROW(A.H + 3);                                                            \
+    else if (blockIdx.x == 0 && threadIdx.x == 448)                                       \
+      PROF_FLUSH_ROW(A.H + 4);                                                            \
   } while (0)
 
 // Proposal of bond p (family order): jump, bosonic action change
 // (cal_energy_difference, :203-228), threshold, Delta (calculate_Delta,
 // :230-245) and the new compact exponential.
-static __device__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
+static __device__ __forceinline__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
   const model_t &md = A.md;
   const size_t tbase = (size_t)md.Nb * (A.t - 1);
   const int fld = A.fbfi[p];
@@ -304,52 +321,23 @@ static __device__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
     }
 }
 
-// The three tiles that fold block k into the window of block k+1, as they
-// were after step k-1: T10 = G[S_{k+1}, S_k] stored transposed
-// (T10t[c * PS_BS + r]), T01 = G[S_k, S_{k+1}] and T11 = G[S_{k+1}, S_{k+1}]
-// row-major; entries outside the blocks are zero.  256 threads (t0 = 0..255)
-// issue all their loads before the first LDS store.
-static __device__ void ps_load_fold_tiles(const ps_args &A, int k, zd *T10t, zd *T01, zd *T11, int t0) {
-  // thread t0 takes row r = t0 % 32 and columns c0 + 8j (j = 0..3) of each tile,
-  // from the compact tile slot written after step k-1 (rows: block k, then k+1)
-  const int n0 = ps_nsites(A, k), n1 = ps_nsites(A, k + 1);
-  const int r = t0 & 31, c0 = t0 >> 5;
-  zd v10[4], v01[4], v11[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int c = c0 + 8 * j;
-    v10[j] = ps_tcol(A, k - 1, 0, c)[PS_BS + r];  // G[S_{k+1}(r), S_k(c)]
-    v01[j] = ps_tcol(A, k - 1, 1, c)[r];          // G[S_k(r), S_{k+1}(c)]
-    v11[j] = ps_tcol(A, k - 1, 1, c)[PS_BS + r];  // G[S_{k+1}(r), S_{k+1}(c)]
-  }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int c = c0 + 8 * j;
-    T10t[c * PS_BS + r] = zsel(r < n1 && c < n0, v10[j]);
-    T01[r * PS_BS + c] = zsel(r < n0 && c < n1, v01[j]);
-    T11[r * PS_BS + c] = zsel(r < n1 && c < n1, v11[j]);
-  }
-}
-
 // --------------------------------------------------------------------------
-// Leader: decisions (wave 0), B (wave 1), next block (waves 2-7), fold.
+// Leader: decisions (wave 0), record (wave 1), next window (waves 2-5),
+// W off its diagonal blocks (wave 6), next proposals (wave 7).
 // --------------------------------------------------------------------------
 struct ps_leader_lds {
-  zd buf[4][PS_BS * PS_BS];     // W, T10, T01, T11 (W and T11 rotate)
+  zd buf[2][PS_BS * PS_BS];     // the window W of block k and that of block k+1 (they rotate)
   zd sU[PS_NB][PS_BS * 4];      // u_m at the block's rows (row*4 + comp)
   zd sV[PS_NB][4 * PS_BS];      // v_m at the block's cols (comp*BS + col)
   zd sX[PS_NB][16];
-  zd sAT[PS_BS * PS_BS];        // A (upper part with diagonal blocks) and T^-1 (strictly lower), stride PS_BS
-  zd sPt[PS_BS * PS_BS];        // P transposed: sPt[g * PS_BS + r] = P[r][g]
-  zd sQ[PS_BS * PS_BS];         // Q (row-major, stride PS_BS)
-  zd sTt[PS_BS * 4];
+  zd nC[2][PS_BS * 4];          // T10[R, S_j] of the accept being folded (row*4 + a), by accept parity
+  zd nR[2][4 * PS_BS];          // T01[S_j, C] (a*BS + col), by accept parity
   zd sM[4][16], sCof[16];
   zd dets[PS_NB];
   ps_prop prop[2][PS_NB];
   int li[PS_NB];
   int npub, done;    // wave 0 -> wave 1: accepts handed over, block decided
-  int nfold, bdone;  // wave 1 -> waves 2-5: accepts folded into A and T^-1, block folded
-  int tiles;         // waves 2-5 that have stored their part of the fold tiles
+  int fbar;          // waves 2-5: barrier count of the accept-by-accept fold
   int wdone;         // wave 6 -> wave 0: accepts whose update of W off its diagonal 4x4 blocks is done
   int nacc, abort;
 };
@@ -357,7 +345,7 @@ struct ps_leader_lds {
 // One wave waits until the LDS count *cnt exceeds have, or until *fin is
 // set; returns the count then seen (have or less means no more will come),
 // or -1 after the clock bound.  The producer sets *fin after its last count.
-static __device__ int ps_lds_progress(const ps_args &A, ps_leader_lds &L, int *cnt, int *fin, int have) {
+static __device__ __forceinline__ int ps_lds_progress(const ps_args &A, ps_leader_lds &L, int *cnt, int *fin, int have) {
   const unsigned long long t0 = wall_clock64();
   for (unsigned it = 1;; ++it) {
     const int np = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -377,13 +365,13 @@ static __device__ int ps_lds_progress(const ps_args &A, ps_leader_lds &L, int *c
   }
 }
 
-// Barrier among the n waves that call it, through an LDS counter that is
-// reset between uses (lane 0 of each wave counts and waits).
-static __device__ void ps_lds_bar(const ps_args &A, ps_leader_lds &L, int *ctr, int n) {
+// Barrier among the n waves that call it, through a monotone LDS counter:
+// the i-th use by each wave waits for the count n * i.
+static __device__ __forceinline__ void ps_lds_bar_to(const ps_args &A, ps_leader_lds &L, int *ctr, int target) {
   if ((threadIdx.x & 63) == 0) {
     __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     const unsigned long long t0 = wall_clock64();
-    for (unsigned it = 1; __hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < n; ++it) {
+    for (unsigned it = 1; __hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target; ++it) {
       if ((it & 63) == 0 && wall_clock64() - t0 > PS_WAIT_TICKS) {
         __hip_atomic_store(&A.sync[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(&A.stats[2], 1ull);
@@ -397,103 +385,99 @@ static __device__ void ps_lds_bar(const ps_args &A, ps_leader_lds &L, int *ctr, 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Wave 1: extend A and T^-1 by accept m of the current block.
-static __device__ void ps_fold_accept(ps_leader_lds &L, int m, int lane) {
-  const int n = 4 * (m + 1);
-  const int jm = L.li[m];
-  zd *__restrict__ sAT = L.sAT;
-  zd *__restrict__ sTt = L.sTt;
-  // T(g, kk) = sel + sum_{m'<m} A[g][4m'+d] v_{m'}[d][S_m(kk)]  (A[g][4m'+d] = 0 unless g/4 <= m')
-  // T^-1[4m + r][g] = - sum_{m'<m} u_{m'}[S_m(r)][d] T^-1[4m'+d][g]  for g < 4m, where
-  // T^-1[4m'+d][g] is the identity on the diagonal block m' = g/4 and stored below it
-  for (int e = lane; e < 4 * n + 4 * 4 * m; e += 64) {
-    if (e < 4 * n) {
-      const int g = e >> 2, kk = e & 3;
-      zd t = zmk(g == 4 * m + kk ? 1.0 : 0.0, 0.0);
-      for (int mp = g >> 2; mp < m; ++mp) {
-        zd av[4], vv[4];
+// Waves 2-5 fold the accepts of block k, one by one, into the tiles of block
+// k+1, as the reference's rank-4 update restricted to them,
+// G <- G + u_m v_m with u_m = G[:, S_j] X_m and v_m = (I - G)[S_j, :]
+// (accept m at bond j, o = 4 (j + 1), R and C the sites of block k+1):
+//   u_m[R] = T10[R, S_j] X_m,  v_m[C] = -T01[S_j, C],
+//   T11 += u_m[R] v_m[C],  T10[R, c] += u_m[R] v_m[c],  T01[r, C] += u_m[r] v_m[C]
+// for the block's later columns c and rows r (>= o), whose u_m[r] and v_m[c]
+// wave 0 has formed from W.  Only the later bonds' columns of T10 and rows of
+// T01 are read again.  Wave w (0..3) keeps the 16 x 16 tile (w / 2, w % 2) of
+// T11, T10 and T01 in MFMA accumulators for the whole block (lane l: rows
+// l/16 + 4i, column l%16); per accept the owners of S_j's column of T10 and
+// row of T01 put them in LDS, and each product is four real FP64 MFMAs of
+// K = 4.  Entries outside the blocks are zero and stay zero.
+struct ps_tiles {
+  double4_t r11, i11, r10, i10, r01, i01;
+};
+
+static __device__ __forceinline__ void ps_tiles_load(const ps_args &A, int k, ps_tiles &T) {
+  const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) - 2, rt = w >> 1, ct = w & 1;
+  const int lr = lane & 15, lk = lane >> 4, col = 16 * ct + lr;
+  const int n0 = ps_nsites(A, k), n1 = ps_nsites(A, k + 1);
+  const zd *c0 = ps_tcol(A, k - 1, 0, col), *c1 = ps_tcol(A, k - 1, 1, col);
+  zd v11[4], v10[4], v01[4];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          av[d] = sAT[g * PS_BS + 4 * mp + d];
-          vv[d] = L.sV[mp][d * PS_BS + 4 * jm + kk];
-        }
-#pragma unroll
-        for (int d = 0; d < 4; ++d) t = zadd(t, zmul(av[d], vv[d]));
-      }
-      sTt[g * 4 + kk] = t;
-    } else {
-      const int e2 = e - 4 * n, nl = 4 * m;
-      const int r = e2 / nl, g = e2 % nl;
-      const int gb = g >> 2;
-      // diagonal block m' = g/4: -u_{m'}[S_m(r)][g % 4]
-      zd t = zneg(L.sU[gb][(4 * jm + r) * 4 + (g & 3)]);
-      for (int mp = gb + 1; mp < m; ++mp) {
-        zd uv[4], tv[4];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          uv[d] = L.sU[mp][(4 * jm + r) * 4 + d];
-          tv[d] = sAT[(4 * mp + d) * PS_BS + g];
-        }
-#pragma unroll
-        for (int d = 0; d < 4; ++d) t = zsub(t, zmul(uv[d], tv[d]));
-      }
-      sAT[(4 * m + r) * PS_BS + g] = t;
-    }
+  for (int i = 0; i < 4; ++i) {
+    const int row = 16 * rt + lk + 4 * i;
+    v11[i] = c1[PS_BS + row];  // G[S_{k+1}(row), S_{k+1}(col)]
+    v10[i] = c0[PS_BS + row];  // G[S_{k+1}(row), S_k(col)]
+    v01[i] = c1[row];          // G[S_k(row), S_{k+1}(col)]
   }
-  wave_sync();
-  // A[:, block m] = T X_m
-  for (int e = lane; e < 4 * n; e += 64) {
-    const int g = e >> 2, c = e & 3;
-    zd tv[4], xv[4];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      tv[kk] = sTt[g * 4 + kk];
-      xv[kk] = L.sX[m][kk + 4 * c];
-    }
-    zd s = zmk(0.0, 0.0);
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) s = zadd(s, zmul(tv[kk], xv[kk]));
-    sAT[g * PS_BS + 4 * m + c] = s;
+  for (int i = 0; i < 4; ++i) {
+    const int row = 16 * rt + lk + 4 * i;
+    const zd a = zsel(row < n1 && col < n1, v11[i]), b = zsel(row < n1 && col < n0, v10[i]),
+             c = zsel(row < n0 && col < n1, v01[i]);
+    T.r11[i] = a.x;
+    T.i11[i] = a.y;
+    T.r10[i] = b.x;
+    T.i10[i] = b.y;
+    T.r01[i] = c.x;
+    T.i01[i] = c.y;
   }
-  wave_sync();
 }
 
-// Waves 2-5 (t2 = 0..255): the contributions of accept m to
-//     P[:, block m] = G[S_{k+1}, S_A] A[:, block m]      (t2 < 128: row t2/4, column t2%4)
-//     Q[block m, :] = T^-1[block m, :] G[S_A, S_{k+1}]   (t2 >= 128: row (t2-128)/32, column t2%32)
-// with T10 stored transposed.  Entries are independent, so no barrier is needed.
-static __device__ void ps_pq_accept(ps_leader_lds &L, int m, int t2, const zd *T10t, const zd *T01) {
-  if (t2 < 128) {
-    const int r = t2 >> 2, c = t2 & 3;
-    zd s0 = zmk(0.0, 0.0), s1 = s0;
-    for (int mb = 0; mb <= m; ++mb) {
-      const int cb = 4 * L.li[mb];
-      zd tv[4], av[4];
+// D += a b for complex operands on FP64 MFMA (four real products)
+static __device__ __forceinline__ void zmfma(double4_t &dr, double4_t &di, zd a, zd b) {
+  dr = mfma_f64(a.x, b.x, dr);
+  di = mfma_f64(a.x, b.y, di);
+  dr = mfma_f64(-a.y, b.y, dr);
+  di = mfma_f64(a.y, b.x, di);
+}
+
+// Step 1 of accept m: the owners put T10[R, S_j] and T01[S_j, C] in LDS.
+static __device__ __forceinline__ void ps_tiles_share(ps_leader_lds &L, int m, const ps_tiles &T) {
+  const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) - 2, rt = w >> 1, ct = w & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int j4 = 4 * L.li[m], par = m & 1;
+  const int a = 16 * ct + lr - j4;
+  if (a >= 0 && a < 4)
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        tv[d] = T10t[(cb + d) * PS_BS + r];
-        av[d] = L.sAT[(4 * mb + d) * PS_BS + 4 * m + c];
-      }
-      s0 = zadd(s0, zadd(zmul(tv[0], av[0]), zmul(tv[1], av[1])));
-      s1 = zadd(s1, zadd(zmul(tv[2], av[2]), zmul(tv[3], av[3])));
-    }
-    L.sPt[(4 * m + c) * PS_BS + r] = zadd(s0, s1);
-  } else {
-    const int e = t2 - 128, rr = e >> 5, c = e & 31;
-    zd s0 = T01[(4 * L.li[m] + rr) * PS_BS + c], s1 = zmk(0.0, 0.0);
-    for (int mb = 0; mb < m; ++mb) {
-      const int cb = 4 * L.li[mb];
-      zd tv[4], ov[4];
+    for (int i = 0; i < 4; ++i) L.nC[par][(16 * rt + lk + 4 * i) * 4 + a] = zmk(T.r10[i], T.i10[i]);
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        tv[d] = L.sAT[(4 * m + rr) * PS_BS + 4 * mb + d];
-        ov[d] = T01[(cb + d) * PS_BS + c];
-      }
-      s0 = zadd(s0, zadd(zmul(tv[0], ov[0]), zmul(tv[1], ov[1])));
-      s1 = zadd(s1, zadd(zmul(tv[2], ov[2]), zmul(tv[3], ov[3])));
-    }
-    L.sQ[(4 * m + rr) * PS_BS + c] = zadd(s0, s1);
+  for (int i = 0; i < 4; ++i) {
+    const int b = 16 * rt + lk + 4 * i - j4;
+    if (b >= 0 && b < 4) L.nR[par][b * PS_BS + 16 * ct + lr] = zmk(T.r01[i], T.i01[i]);
   }
+}
+
+// Step 2 of accept m: the three rank-4 products on the wave's tiles.
+static __device__ __forceinline__ void ps_tiles_fold(ps_leader_lds &L, int m, int nsk, ps_tiles &T) {
+  const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) - 2, rt = w >> 1, ct = w & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int o = 4 * (L.li[m] + 1), par = m & 1;
+  const int row = 16 * rt + lr, col = 16 * ct + lr;
+  // A operands: u_m[R] at (row, lk) and, for T01, wave 0's u_m at the block's rows
+  zd u = zmk(0.0, 0.0);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) u = zadd(u, zmul(L.nC[par][row * 4 + a], L.sX[m][a + 4 * lk]));
+  const zd ub = zsel(row >= o && row < nsk, L.sU[m][row * 4 + lk]);
+  // B operands: v_m[C] at (lk, col) and, for T10, wave 0's v_m at the block's columns
+  const zd v = zneg(L.nR[par][lk * PS_BS + col]);
+  const zd vb = zsel(col >= o && col < nsk, L.sV[m][lk * PS_BS + col]);
+  zmfma(T.r11, T.i11, u, v);
+  zmfma(T.r10, T.i10, u, vb);
+  zmfma(T.r01, T.i01, ub, v);
+}
+
+// The wave's tile of T11 (the next window) to LDS, row-major.
+static __device__ __forceinline__ void ps_tiles_store(zd *W, const ps_tiles &T) {
+  const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) - 2, rt = w >> 1, ct = w & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) W[(16 * rt + lk + 4 * i) * PS_BS + 16 * ct + lr] = zmk(T.r11[i], T.i11[i]);
 }
 
 // Wave 6: W += u_m v_m on the later rows and columns of accept m, except the
@@ -504,7 +488,7 @@ static __device__ void ps_pq_accept(ps_leader_lds &L, int m, int t2, const zd *T
 // and V[l / 16][l % 16] and holds rows l / 16 + 4 i of column l % 16.  Every
 // entry of W has one writer, in accept order, so the result does not depend
 // on how the two waves interleave.
-static __device__ void ps_w_offdiag(ps_leader_lds &L, zd *W, int m, int nsk) {
+static __device__ __forceinline__ void ps_w_offdiag(ps_leader_lds &L, zd *W, int m, int nsk) {
   const int lane = threadIdx.x & 63;
   const int o = 4 * (L.li[m] + 1);
   if (o >= nsk) return;
@@ -538,13 +522,13 @@ static __device__ void ps_w_offdiag(ps_leader_lds &L, zd *W, int m, int nsk) {
   wave_sync();
 }
 
-static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
+static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const model_t &md = A.md;
   const int npla = md.npla, nb = A.nb, nblk = A.nblk;
   const size_t tbase = (size_t)md.Nb * (A.t - 1);
   PROF_T0();
-  int iW = 0, i10 = 1, i01 = 2, i11 = 3;
+  int iW = 0, i11 = 1;
   bool ok = true;
   // a backward pass wraps every column with the exponentials before the
   // update; none may change until all helpers have read them
@@ -555,9 +539,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     L.abort = 0;
     L.npub = 0;
     L.done = 0;
-    L.nfold = 0;
-    L.bdone = 0;
-    L.tiles = 0;
+    L.fbar = 0;
     L.wdone = 0;
     L.nacc = ps_wait(A, ps_pub(A, -1, 0), ps_owners(A, 0), nullptr, 0) ? 1 : 0;
   }
@@ -730,89 +712,55 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
         ++m;
         ++total_acc;
       }
+      // The accepted exponentials are read by the helpers' forward wrap after
+      // the last block (acc flags and fields only by later launches): this
+      // wave's stores are complete before wave 1 publishes the last block.
+      if (!has_next) ps_release();
       if (lane == 0) {
         L.nacc = m;
         __hip_atomic_store(&L.done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
-      ps_release();  // acc flags, fields and exponentials are complete before block k is published
       PROF_ACC(0);
     } else if (wave == 1) {
-      // ---- A and T^-1 of block k, accept by accept ----
+      // ---- the record of block k, accept by accept: wave 0's 4 x 4 blocks
+      // X_m, T[m][m'] = u_m'[S_m] and X_m' V[m'][m] = X_m' v_m'[S_m] (m' < m),
+      // which are final once accept m is handed over ----
       int md_ = 0;
+      ps_rec *R = A.rec + k;
       for (;;) {
         PROF_ACC(0);
         const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
         PROF_ACC(0);
         if (np <= md_) break;
         for (; md_ < np; ++md_) {
-          ps_fold_accept(L, md_, lane);
-          if (lane == 0) __hip_atomic_store(&L.nfold, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-          // the record's entries that accept md_ fixed: T^-1 rows 4 md_..4 md_+3
-          // left of the diagonal block, A columns 4 md_..4 md_+3 down to row
-          // 4 md_+3; later accepts leave them unchanged
-          ps_rec *R = A.rec + k;
-          const int m4 = 4 * md_;
-          for (int e = lane; e < 4 * m4 + 4 * (m4 + 4); e += 64) {
-            const int r = e < 4 * m4 ? m4 + e / m4 : (e - 4 * m4) >> 2;
-            const int c = e < 4 * m4 ? e % m4 : m4 + ((e - 4 * m4) & 3);
-            st_coh(&R->AT[r * PS_BS + c], L.sAT[r * PS_BS + c]);
+          const int m = md_, j4 = 4 * L.li[m];
+          if (lane == 0) st_coh_i(&R->li[m], L.li[m]);
+          for (int e = lane; e < 16 * (2 * m + 1); e += 64) {
+            const int blk = e >> 4, r = (e >> 2) & 3, c = e & 3;
+            if (blk == 0) {
+              st_coh(&R->AT[(4 * m + r) * PS_BS + 4 * m + c], L.sX[m][r + 4 * c]);
+            } else if (blk <= m) {
+              const int mp = blk - 1;
+              st_coh(&R->AT[(4 * m + r) * PS_BS + 4 * mp + c], L.sU[mp][(j4 + r) * 4 + c]);
+            } else {
+              const int mp = blk - 1 - m;
+              zd xv = zmk(0.0, 0.0);
+#pragma unroll
+              for (int e2 = 0; e2 < 4; ++e2) xv = zadd(xv, zmul(L.sX[mp][r + 4 * e2], L.sV[mp][e2 * PS_BS + j4 + c]));
+              st_coh(&R->AT[(4 * mp + r) * PS_BS + 4 * m + c], xv);
+            }
           }
         }
         PROF_ACC(1);
       }
-      if (lane == 0) __hip_atomic_store(&L.bdone, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      // the record's accepted indices now (its A and T^-1 went out accept by
-      // accept); its flag after the barrier
+      // ---- publish block k as soon as it is decided: the record went out
+      // accept by accept; its stores are complete, then the flag.  The
+      // helpers do not wait for this workgroup's fold of the next window. ----
       const int a = L.nacc;
-      ps_rec *R = A.rec + k;
-      if (lane < a) st_coh_i(&R->li[lane], L.li[lane]);
-      PROF_ACC(4);
-    } else if (has_next && wave <= 5) {
-      // ---- waves 2..5: the fold tiles after step k-1, then P and Q ----
-      bool wok = true;
-      if (lane == 0)
-        wok = ps_wait(A, ps_pub(A, k - 1, 0), ps_owners(A, k), ps_pub(A, k - 1, 1), ps_owners(A, k + 1)) != 0;
-      wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
-      PROF_ACC(1);
-      if (wok) {
-        ps_load_fold_tiles(A, k, L.buf[i10], L.buf[i01], L.buf[i11], tid - 128);
-        ps_lds_bar(A, L, &L.tiles, 4);  // P and Q read all of the tiles
-        PROF_ACC(7);
-        int md_ = 0;
-        for (;;) {
-          const int np = ps_lds_progress(A, L, &L.nfold, &L.bdone, md_);
-          if (np <= md_) break;
-          for (; md_ < np; ++md_) ps_pq_accept(L, md_, tid - 128, L.buf[i10], L.buf[i01]);
-        }
-      } else if (lane == 0) {
-        L.abort = 1;
+      if (!L.abort) {
+        ps_release();
+        if (lane == 0) ps_st(ps_flag(A, k), (unsigned long long)(a + 1));
       }
-      PROF_ACC(5);
-    } else if (wave == 6) {
-      // ---- W off its diagonal blocks, accept by accept ----
-      int md_ = 0;
-      for (;;) {
-        const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
-        if (np <= md_) break;
-        for (; md_ < np; ++md_) {
-          ps_w_offdiag(L, L.buf[iW], md_, nsk);
-          if (lane == 0) __hip_atomic_store(&L.wdone, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-      }
-    } else if (has_next && wave == 7) {
-      // ---- wave 7: the next block's proposals ----
-      const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
-      if (lane < nb1) ps_propose(A, p1 + lane, L.prop[(k + 1) & 1][lane]);
-    }
-    if (tid == 0 && ps_ld(&A.sync[2]) != 0) L.abort = 1;
-    __syncthreads();
-    if (L.abort) { ok = false; break; }
-    PROF_ACC(2);
-    const int a = L.nacc;
-    if (wave == 1) {
-      // ---- publish block k: the record stores are complete, then the flag ----
-      ps_release();
-      if (lane == 0) ps_st(ps_flag(A, k), (unsigned long long)(a + 1));
       // ln_cw += log(real(R_elec)) per accept (src/mod_update.f90:142)
       double lv = 0.0;
       if (lane < a) {
@@ -825,42 +773,58 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
       lv = wave_sum(lv);
       if (lane == 0 && a > 0) atomicAdd(A.lncw_acc, lv);
       PROF_ACC(4);
+    } else if (has_next && wave <= 5) {
+      // ---- waves 2..5: the fold tiles after step k-1, then block k's accepts one by one ----
+      bool wok = true;
+      if (lane == 0)
+        wok = ps_wait(A, ps_pub(A, k - 1, 0), ps_owners(A, k), ps_pub(A, k - 1, 1), ps_owners(A, k + 1)) != 0;
+      wok = __shfl(wok ? 1 : 0, 0, 64) != 0;
+      PROF_ACC(1);
+      if (wok) {
+        ps_tiles T;
+        ps_tiles_load(A, k, T);
+        PROF_ACC(7);
+        int md_ = 0;
+        for (;;) {
+          const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
+          if (np <= md_) break;
+          for (; md_ < np; ++md_) {
+            ps_tiles_share(L, md_, T);
+            // one barrier per accept: the shared column and row are double-buffered by parity
+            ps_lds_bar_to(A, L, &L.fbar, 4 * (md_ + 1));
+            ps_tiles_fold(L, md_, nsk, T);
+          }
+        }
+        ps_tiles_store(L.buf[i11], T);
+      } else if (lane == 0) {
+        L.abort = 1;
+      }
+      PROF_ACC(5);
+    } else if (wave == 6) {
+      // ---- W off its diagonal blocks, accept by accept ----
+      int md_ = 0;
+      for (;;) {
+        const int np = ps_lds_progress(A, L, &L.npub, &L.done, md_);
+        if (np <= md_) break;
+        PROF_ACC(0);
+        for (; md_ < np; ++md_) {
+          ps_w_offdiag(L, L.buf[iW], md_, nsk);
+          if (lane == 0) __hip_atomic_store(&L.wdone, md_ + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        PROF_ACC(1);
+      }
+      PROF_ACC(0);
+    } else if (has_next && wave == 7) {
+      // ---- wave 7: the next block's proposals ----
+      const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
+      if (lane < nb1) ps_propose(A, p1 + lane, L.prop[(k + 1) & 1][lane]);
+      PROF_ACC(1);
     }
+    if (tid == 0 && ps_ld(&A.sync[2]) != 0) L.abort = 1;
+    __syncthreads();
+    if (L.abort) { ok = false; break; }
+    PROF_ACC(2);
     if (!has_next) break;
-    zd *T11 = L.buf[i11];
-    if (a > 0 && wave >= 2 && wave <= 5) {
-      // ---- fold block k into the next window: T11 -= P Q, K = 4a ----
-      // on FP64 MFMA, one 16 x 16 tile per wave; lane l supplies A[l%16][l/16]
-      // and B[l/16][l%16], and holds rows l/16 + 4i of column l%16 of the result.
-      const int tt = wave - 2;
-      const int rt = tt >> 1, ct = tt & 1;
-      const int lr = lane & 15, lk = lane >> 4;
-      double4_t dre0, dim0, dre1 = {0.0, 0.0, 0.0, 0.0}, dim1 = dre1;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const zd t = T11[(16 * rt + lk + 4 * i) * PS_BS + 16 * ct + lr];
-        dre0[i] = t.x;
-        dim0[i] = t.y;
-      }
-      for (int ks = 0; ks < a; ks += 2) {
-        const int k2 = min(ks + 1, a - 1);
-        const zd a0 = L.sPt[(4 * ks + lk) * PS_BS + 16 * rt + lr];
-        const zd b0 = L.sQ[(4 * ks + lk) * PS_BS + 16 * ct + lr];
-        const zd a1 = zsel(ks + 1 < a, L.sPt[(4 * k2 + lk) * PS_BS + 16 * rt + lr]);
-        const zd b1 = L.sQ[(4 * k2 + lk) * PS_BS + 16 * ct + lr];
-        dre0 = mfma_f64(-a0.x, b0.x, dre0);
-        dim0 = mfma_f64(-a0.x, b0.y, dim0);
-        dre1 = mfma_f64(-a1.x, b1.x, dre1);
-        dim1 = mfma_f64(-a1.x, b1.y, dim1);
-        dre0 = mfma_f64(a0.y, b0.y, dre0);
-        dim0 = mfma_f64(-a0.y, b0.x, dim0);
-        dre1 = mfma_f64(a1.y, b1.y, dre1);
-        dim1 = mfma_f64(-a1.y, b1.x, dim1);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        T11[(16 * rt + lk + 4 * i) * PS_BS + 16 * ct + lr] = zmk(dre0[i] + dre1[i], dim0[i] + dim1[i]);
-    }
     if (wave == 0)
       PROF_ACC(1);
     else
@@ -868,9 +832,7 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
     if (tid == 0) {
       L.npub = 0;
       L.done = 0;
-      L.nfold = 0;
-      L.bdone = 0;
-      L.tiles = 0;
+        L.fbar = 0;
       L.wdone = 0;
     }
     __syncthreads();
@@ -891,16 +853,17 @@ static __device__ void ps_leader(const ps_args &A, ps_leader_lds &L) {
 // Helpers: own columns, wraps, block steps.
 // --------------------------------------------------------------------------
 struct ps_helper_hdr {
-  zd sAT[PS_BS * PS_BS];       // A and T^-1 of the block (packed as in the record, stride PS_BS)
-  zd sB[PS_BS * PS_BS];        // v = T^-1 w per own column (stride PS_BS)
+  zd sAT[PS_BS * PS_BS];       // the block's record (X, T and V blocks, stride PS_BS)
+  zd sB[PS_BS * PS_BS];        // v, then s, per own column (stride PS_BS)
   int li[PS_NB];
+  int pos[PS_NB];              // position of block-local bond b among the accepts, or -1
   int a, ok, sink;
 };
 // after the header: sw, sy [4 bph][PS_BS], sC [4 bph][Ns], then the family's
 // sites [4 npla] (int)
 
 // Own columns: row r of local column q at sC[q * Ns + r].
-static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bool forward) {
+static __device__ __forceinline__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bool forward) {
   const int Ns = A.md.Ns, npla = A.md.npla, tid = threadIdx.x;
   const ekv_t *ek = A.ekv + ((size_t)(A.t - 1) * 4 + A.f) * npla;
   const int nbo = b1 - b0;
@@ -934,9 +897,12 @@ static __device__ void ps_wrap_cols(const ps_args &A, zd *sC, int b0, int b1, bo
   __syncthreads();
 }
 
-// After step kstep the slot holds the columns of blocks kstep+1, kstep+2 and
-// the tile slot the same columns at the rows of those two blocks.
-static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, const int *sS, int b0, int b1, int kstep) {
+// After step kstep the tile slot holds the columns of blocks kstep+1, kstep+2
+// at the rows of those two blocks (the leader's fold tiles), and the slot the
+// columns of block kstep+1 (the pivot columns of step kstep+1).  The tiles go
+// out first: the leader waits for them, the helpers' next step for the slot.
+static __device__ __forceinline__ void ps_publish_tiles(const ps_args &A, const zd *sC, const int *sS, int b0, int b1,
+                                                       int kstep) {
   const int Ns = A.md.Ns, tid = threadIdx.x;
   for (int b = b0; b < b1; ++b) {
     const int blk = b / A.nb;
@@ -944,34 +910,120 @@ static __device__ void ps_publish_cols(const ps_args &A, const zd *sC, const int
     if (mb < 0 || mb > 1) continue;
     const int cl0 = 4 * (b - blk * A.nb);
     const zd *col = sC + 4 * (b - b0) * Ns;
-    for (int e = tid; e < 4 * Ns; e += PS_THREADS) {
-      const int c = e / Ns, r = e % Ns;
-      st_coh(ps_scol(A, kstep, mb, cl0 + c) + r, col[c * Ns + r]);
-    }
     for (int e = tid; e < 4 * 2 * PS_BS; e += PS_THREADS) {
       const int c = e >> 6, i = e & 63, j = kstep + 1 + (i >> 5), r = i & 31;
       if (j < A.nblk && r < ps_nsites(A, j)) st_coh(ps_tcol(A, kstep, mb, cl0 + c) + i, col[c * Ns + sS[4 * A.nb * j + r]]);
     }
   }
 }
+static __device__ __forceinline__ void ps_publish_slot(const ps_args &A, const zd *sC, int b0, int b1, int kstep) {
+  const int Ns = A.md.Ns, tid = threadIdx.x;
+  for (int b = b0; b < b1; ++b) {
+    const int blk = b / A.nb;
+    if (blk != kstep + 1) continue;
+    const int cl0 = 4 * (b - blk * A.nb);
+    const zd *col = sC + 4 * (b - b0) * Ns;
+    for (int e = tid; e < 4 * Ns; e += PS_THREADS) {
+      const int c = e / Ns, r = e % Ns;
+      st_coh(ps_scol(A, kstep, 0, cl0 + c) + r, col[c * Ns + r]);
+    }
+  }
+}
+// Publish after step kstep (all threads): tiles, their count, slot, its count.
+static __device__ __forceinline__ void ps_announce(const ps_args &A, int b0, int b1, int kstep);
+static __device__ __forceinline__ void ps_publish_cols(const ps_args &A, const zd *sC, const int *sS, int b0, int b1,
+                                                      int kstep) {
+  bool tiles = false, slot = false;
+  for (int b = b0; b < b1; ++b) {
+    const int mb = b / A.nb - (kstep + 1);
+    tiles = tiles || mb == 0 || mb == 1;
+    slot = slot || mb == 0;
+  }
+  if (!tiles) return;  // uniform
+  ps_publish_tiles(A, sC, sS, b0, b1, kstep);
+  ps_release();
+  __syncthreads();
+  if (threadIdx.x == 0) ps_announce(A, b0, b1, kstep);
+  if (slot) {
+    ps_publish_slot(A, sC, b0, b1, kstep);
+    ps_release();
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(ps_spub(A, kstep), 1ull);
+  }
+}
 
 // One thread: count this helper among the publishers of the blocks
 // kstep+1, kstep+2 that it owns bonds of (after its stores are complete).
-static __device__ void ps_announce(const ps_args &A, int b0, int b1, int kstep) {
-  bool own[2] = {false, false};
+static __device__ __forceinline__ void ps_announce(const ps_args &A, int b0, int b1, int kstep) {
+  bool own0 = false, own1 = false;
   for (int b = b0; b < b1; ++b) {
     const int mb = b / A.nb - (kstep + 1);
-    if (mb == 0 || mb == 1) own[mb] = true;
+    own0 = own0 || mb == 0;
+    own1 = own1 || mb == 1;
   }
-  for (int mb = 0; mb < 2; ++mb)
-    if (own[mb]) atomicAdd(ps_pub(A, kstep, mb), 1ull);
+  if (own0) atomicAdd(ps_pub(A, kstep, 0), 1ull);
+  if (own1) atomicAdd(ps_pub(A, kstep, 1), 1ull);
+}
+
+// y[q] = (y_0, .., y_{a-1}) of the own columns q from the record (the 4 x 4
+// blocks X, T and X V of ps_rec):
+//     v_m = w_m - sum_{m'<m} T[m][m'] v_m',    w_m = (I - G)[S_m, q],
+//     y_m = X_m v_m + sum_{m'>m} (X_m V[m][m']) y_m'   (m = a-1 .. 0),
+// so that G[:, q] += G[:, S_A] y[q] is the block's rank-4 updates in turn.
+// Wave w takes columns 4w..4w+3; lane (ql, c, d) = (l / 16, l / 4 % 4, l % 4)
+// forms component c of column 4w+ql, its sums over d reduced in the quad.
+// No workgroup barrier: the waves' columns are independent.  v is kept at
+// sv[q * PS_BS + 4m + c], y at sy[q * PS_BS + 4m + c].
+static __device__ __forceinline__ void ps_helper_y(const ps_args &A, const ps_helper_hdr &Hh, int k, int a, int ncol,
+                                                   int b0, const zd *sC, const int *sS, zd *sv, zd *sy) {
+  const int Ns = A.md.Ns, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (4 * wv >= ncol) return;  // uniform
+  const int ql = lane >> 4, c = (lane >> 2) & 3, d = lane & 3, q = 4 * wv + ql;
+  const bool qok = q < ncol, lead = qok && d == 0;
+  const int *Sk = sS + 4 * A.nb * k;
+  const int siteq = qok ? sS[4 * b0 + q] : -1;
+  const zd *M = Hh.sAT;
+  for (int m = 0; m < a; ++m) {
+    zd acc = zmk(0.0, 0.0);
+    if (lead) {
+      const int st = Sk[4 * Hh.li[m] + c];
+      acc = zsub(zmk(siteq == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
+    }
+    zd acc1 = zmk(0.0, 0.0);
+    if (qok)
+      for (int mp = 0; mp < m; mp += 2) {
+        acc = zsub(acc, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], sv[q * PS_BS + 4 * mp + d]));
+        if (mp + 1 < m) acc1 = zsub(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], sv[q * PS_BS + 4 * mp + 4 + d]));
+      }
+    acc = zadd(acc, acc1);
+    acc.x = quad_sum(acc.x);
+    acc.y = quad_sum(acc.y);
+    if (lead) sv[q * PS_BS + 4 * m + c] = acc;
+    wave_sync();
+  }
+  // y_m = X_m v_m + sum_{m'>m} (X_m V[m][m']) y_m'  (the record holds X_m V above the diagonal)
+  for (int m = a - 1; m >= 0; --m) {
+    zd acc0 = zmk(0.0, 0.0), acc1 = acc0;
+    if (qok) {
+      acc0 = zmul(M[(4 * m + c) * PS_BS + 4 * m + d], sv[q * PS_BS + 4 * m + d]);
+      for (int mp = m + 1; mp < a; mp += 2) {
+        acc1 = zadd(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], sy[q * PS_BS + 4 * mp + d]));
+        if (mp + 1 < a) acc0 = zadd(acc0, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], sy[q * PS_BS + 4 * mp + 4 + d]));
+      }
+    }
+    zd yv = zadd(acc0, acc1);
+    yv.x = quad_sum(yv.x);
+    yv.y = quad_sum(yv.y);
+    if (lead) sy[q * PS_BS + 4 * m + c] = yv;
+    wave_sync();
+  }
 }
 
 // G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own columns, four at a
 // time.  All pivot loads of a row are issued before the first use (entries
 // past 4a read one fixed word, a single request per wave, and are not used);
 // y is staged in registers four pivots at a time.
-static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4, int ncol,
+static __device__ __forceinline__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4, int ncol,
                                        const zd *sy, zd *sC) {
   const int Ns = A.md.Ns;
   constexpr int HB = PS_BS / 2;
@@ -1021,7 +1073,7 @@ static __device__ void ps_pivot_update(const ps_args &A, const ps_helper_hdr &Hh
 // Lane l supplies P[t0 + l%16][g0 + l/16] and Y[g0 + l/16][l%16] and holds C
 // rows l/16 + 4i of column l%16.  All pivot loads of a tile are issued
 // before the first product.
-static __device__ void ps_pivot_update_mfma(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4,
+static __device__ __forceinline__ void ps_pivot_update_mfma(const ps_args &A, const ps_helper_hdr &Hh, int kstep, int n4,
                                             const zd *sy, zd *sC) {
   const int Ns = A.md.Ns, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -1061,8 +1113,8 @@ static __device__ void ps_pivot_update_mfma(const ps_args &A, const ps_helper_hd
   }
 }
 
-template <int BPH>
-static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy, int *sS) {
+template <int BPH, bool PRE>
+static __device__ __forceinline__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd *sw, zd *sy, int *sS) {
   constexpr int NC = 4 * BPH;
   const int tid = threadIdx.x;
   const model_t &md = A.md;
@@ -1081,13 +1133,8 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
   __syncthreads();
   if (A.wrap == 2) ps_wrap_cols(A, sC, b0, b1, false);
   ps_publish_cols(A, sC, sS, b0, b1, -1);
-  ps_release();
-  __syncthreads();
-  if (tid == 0) {
-    ps_announce(A, b0, b1, -1);
-    // this workgroup has read the exponentials its backward wrap needs
-    atomicAdd(ps_wrapped(A), 1ull);
-  }
+  // this workgroup has read the exponentials its backward wrap needs
+  if (tid == 0) atomicAdd(ps_wrapped(A), 1ull);
   PROF_ACC(0);
   for (int k = 0; k < A.nblk; ++k) {
     // Block k's columns are in the slot once their owners have published
@@ -1095,10 +1142,21 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     // Meanwhile the helpers of an XCD (workgroups are spread over the 8 XCDs
     // round-robin) touch those columns in slices, so that the pivot loads of
     // step k hit in L2.
-    if (tid == 0) Hh.ok = ps_wait(A, ps_pub(A, k - 1, 0), ps_owners(A, k), nullptr, 0) != 0;
+    if (tid == 0) Hh.ok = ps_wait(A, ps_spub(A, k - 1), ps_owners(A, k), nullptr, 0) != 0;
     __syncthreads();
     if (!Hh.ok) break;
-    {
+    // With one bond per helper and one row per thread, each thread loads its
+    // row of all the block's columns G_{k-1}[r, S_k] into registers while the
+    // leader decides the block, so that the update after the decision has no
+    // memory latency left (entries past the block read column 0 and are not
+    // used).  Otherwise the helpers of an XCD touch the columns in L2.
+    constexpr bool pre = PRE;
+    zd Pk[PRE ? PS_BS : 1];
+    if constexpr (pre) {
+      const int nsk = ps_nsites(A, k), r = tid < Ns ? tid : 0;
+#pragma unroll
+      for (int g = 0; g < (PRE ? PS_BS : 1); ++g) Pk[g] = ps_scol(A, k - 1, 0, g < nsk ? g : 0)[r];
+    } else {
       // one load per 128-byte line (8 entries of a column): line ln goes to
       // the helper whose slice is ln % 12
       const int nsk = ps_nsites(A, k), slice = (blockIdx.x >> 3) % 12, nl = (Ns + 7) >> 3;
@@ -1121,72 +1179,54 @@ static __device__ void ps_helper(const ps_args &A, ps_helper_hdr &Hh, zd *sC, zd
     if (a > 0) {
       const ps_rec *R = A.rec + k;
       if (tid < a) Hh.li[tid] = R->li[tid];
+      if (tid < PS_NB) Hh.pos[tid] = -1;
       for (int e = tid; e < n4 * n4; e += PS_THREADS) {
         const int r = e / n4, c = e % n4;
         Hh.sAT[r * PS_BS + c] = R->AT[r * PS_BS + c];
       }
       __syncthreads();
       PROF_ACC(2);
-      // w[q][g] = e_q(S_g) - G_k[S_g, q], then y[q] = B w[q] = A (T^-1 w[q]) as
-      // two block-triangular products (no n4 x n4 product B = A T^-1):
-      //   v[q][g] = w[q][g] + sum_{h/4 < g/4} T^-1[g][h] w[q][h]
-      //   y[q][g] = sum_{h/4 >= g/4} A[g][h] v[q][h]
-      const int *Sk = sS + 4 * A.nb * k;
-      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
-        const int q = e / n4, g = e % n4;
-        const int st = Sk[4 * Hh.li[g >> 2] + (g & 3)];
-        sw[q * PS_BS + g] = zsub(zmk(sS[4 * b0 + q] == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
-      }
-      __syncthreads();
-      zd *sv = Hh.sB;  // v[q][g] at sv[q * PS_BS + g]
-      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
-        const int q = e / n4, g = e % n4;
-        const int hend = 4 * (g >> 2);
-        zd v0 = sw[q * PS_BS + g], v1 = zmk(0.0, 0.0);
-        for (int hh = 0; hh < hend; hh += 4) {
-          zd tv[4], wv[4];
-#pragma unroll
-          for (int d = 0; d < 4; ++d) {
-            tv[d] = Hh.sAT[g * PS_BS + hh + d];
-            wv[d] = sw[q * PS_BS + hh + d];
-          }
-          v0 = zfma(zfma(v0, tv[0], wv[0]), tv[1], wv[1]);
-          v1 = zfma(zfma(v1, tv[2], wv[2]), tv[3], wv[3]);
-        }
-        sv[q * PS_BS + g] = zadd(v0, v1);
-      }
-      __syncthreads();
-      for (int e = tid; e < ncol * n4; e += PS_THREADS) {
-        const int q = e / n4, g = e % n4;
-        zd y0 = zmk(0.0, 0.0), y1 = y0;
-        for (int hh = 4 * (g >> 2); hh < n4; hh += 4) {
-          zd av[4], vv[4];
-#pragma unroll
-          for (int d = 0; d < 4; ++d) {
-            av[d] = Hh.sAT[g * PS_BS + hh + d];
-            vv[d] = sv[q * PS_BS + hh + d];
-          }
-          y0 = zfma(zfma(y0, av[0], vv[0]), av[1], vv[1]);
-          y1 = zfma(zfma(y1, av[2], vv[2]), av[3], vv[3]);
-        }
-        sy[q * PS_BS + g] = zadd(y0, y1);
-      }
+      if (tid < a) Hh.pos[Hh.li[tid]] = tid;
+      ps_helper_y(A, Hh, k, a, ncol, b0, sC, sS, Hh.sB, sy);
       __syncthreads();
       PROF_ACC(3);
       // G[:, q] += G_k[:, S_A] y[q]; pivot columns from the slot of step k-1
-      if constexpr (BPH == 4)
+      if constexpr (BPH == 4) {
         ps_pivot_update_mfma(A, Hh, k - 1, n4, sy, sC);
-      else
+      } else if constexpr (pre) {
+        // the accepted bonds' columns from the registers (uniform branches)
+        if (tid < Ns) {
+          // one accumulator per (column, pivot within the bond): short FMA chains
+          zd acc[NC][4];
+#pragma unroll
+          for (int q = 0; q < NC; ++q)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) acc[q][d] = zmk(0.0, 0.0);
+          int pos[PS_NB];
+#pragma unroll
+          for (int b = 0; b < PS_NB; ++b) pos[b] = Hh.pos[b];
+#pragma unroll
+          for (int b = 0; b < (PRE ? PS_NB : 0); ++b) {
+            const int m = pos[b];
+            if (m >= 0) {
+#pragma unroll
+              for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int q = 0; q < NC; ++q) acc[q][d] = zfma(acc[q][d], Pk[4 * b + d], sy[q * PS_BS + 4 * m + d]);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NC; ++q)
+            sC[q * Ns + tid] = zadd(sC[q * Ns + tid], zadd(zadd(acc[q][0], acc[q][1]), zadd(acc[q][2], acc[q][3])));
+        }
+      } else {
         ps_pivot_update(A, Hh, k - 1, n4, NC, sy, sC);
+      }
       __syncthreads();
     }
     PROF_ACC(5);
     ps_publish_cols(A, sC, sS, b0, b1, k);
-    ps_release();
-    __syncthreads();
     PROF_ACC(6);
-    if (tid == 0) ps_announce(A, b0, b1, k);
-    PROF_ACC(7);
   }
   if (A.wrap == 1) ps_wrap_cols(A, sC, b0, b1, true);
   for (int e = tid; e < ncol * Ns; e += PS_THREADS) {
@@ -1213,12 +1253,15 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
     zd *sy = sw + 4 * A.bph * PS_BS;
     zd *sC = sy + 4 * A.bph * PS_BS;
     int *sS = reinterpret_cast<int *>(sC + 4 * A.bph * A.md.Ns);
-    if (A.bph == 1)
-      ps_helper<1>(A, Hh, sC, sw, sy, sS);
+    // one bond per helper with one row per thread keeps the pivot rows in registers
+    if (A.bph == 1 && A.md.Ns <= PS_THREADS)
+      ps_helper<1, true>(A, Hh, sC, sw, sy, sS);
+    else if (A.bph == 1)
+      ps_helper<1, false>(A, Hh, sC, sw, sy, sS);
     else if (A.bph == 2)
-      ps_helper<2>(A, Hh, sC, sw, sy, sS);
+      ps_helper<2, false>(A, Hh, sC, sw, sy, sS);
     else
-      ps_helper<4>(A, Hh, sC, sw, sy, sS);
+      ps_helper<4, false>(A, Hh, sC, sw, sy, sS);
   }
   // the last workgroup to finish clears the sync words for the next launch
   __syncthreads();
@@ -1232,7 +1275,10 @@ __global__ __launch_bounds__(PS_THREADS) void k_family_pass(ps_args A) {
         ps_st(ps_pub(A, k, 0), 0ull);
         ps_st(ps_pub(A, k, 1), 0ull);
       }
-      for (int k = 0; k < A.nblk; ++k) ps_st(ps_flag(A, k), 0ull);
+      for (int k = 0; k < A.nblk; ++k) {
+        ps_st(ps_flag(A, k), 0ull);
+        ps_st(ps_spub(A, k - 1), 0ull);
+      }
       ps_st(&A.sync[1], 0ull);
     }
   }

@@ -1524,10 +1524,10 @@ extern "C" int pqmc_set_counters(pqmc_handle *h, const int64_t *in2, double err_
 
 // Phase timers of the family pass (debug): enable != 0 allocates and
 // zeroes them; out (8*P values, may be NULL) receives microseconds.  Rows:
-// leader wave 0, the H helpers, leader wave 1, leader wave 2.
+// leader wave 0, the H helpers, leader waves 1, 2, 6 and 7.
 extern "C" int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P) {
   if (!h) return fail(PQMC_EINVAL, "null handle");
-  const int G = h->H + 3;
+  const int G = h->H + 5;
   if (P) *P = G;
   if (out && h->d_prof) {
     std::vector<unsigned long long> v(8 * G);
