@@ -647,7 +647,7 @@ extern "C" hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s)
 // check is skipped, and so it is here.  A non-finite new value counts as +inf.
 __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                        int tr_weight, int ncopy, zd *lncw_out, const double *acc, unsigned long long *drift_call,
-                       unsigned long long *drift_run) {
+                       unsigned long long *drift_run, int conjq) {
   __shared__ double sr[4], si[4];
   const int tid = threadIdx.x;
   double ar = 0.0, ai = 0.0;
@@ -666,7 +666,9 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
   }
   __syncthreads();
   if (tid == 0) {
-    const double ldr = sr[0] + sr[1] + sr[2] + sr[3], ldi_raw = si[0] + si[1] + si[2] + si[3];
+    // conjq: the QR was of M^H, whose determinant is the conjugate of det M
+    const double ldr = sr[0] + sr[1] + sr[2] + sr[3], ldi_q = si[0] + si[1] + si[2] + si[3];
+    const double ldi_raw = conjq ? -ldi_q : ldi_q;
     double dr = 0.0, di = 0.0;
     for (int b = 0; b < nblk; ++b) {
       dr += dlog[b].x;
@@ -692,9 +694,9 @@ __global__ void k_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn
 
 extern "C" hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                                 int tr_weight, int ncopy, zd *out, const double *acc, unsigned long long *drift_call,
-                                unsigned long long *drift_run, hipStream_t s) {
+                                unsigned long long *drift_run, int conjq, hipStream_t s) {
   hipLaunchKernelGGL(k_lncw, dim3(1), dim3(256), 0, s, ne, betaM, tauM, vn2M, dlog, nblk, tr_weight, ncopy, out, acc,
-                     drift_call, drift_run);
+                     drift_call, drift_run, conjq);
   return hipGetLastError();
 }
 

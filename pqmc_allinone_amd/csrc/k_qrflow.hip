@@ -68,13 +68,42 @@ static __device__ __forceinline__ zd rdlane_zq(zd v, int l) {
 
 // Wait (all threads) until flag word w holds this launch's epoch.  Returns
 // false when the launch is aborted.
-static __device__ bool qf_wait(const qf_args &a, int w, int *sStop) {
+static __device__ __forceinline__ bool qf_wait(const qf_args &a, int w, int *sStop) {
   if (threadIdx.x == 0) {
     unsigned long long *f = &a.sync[w];
     const unsigned long long t0 = wall_clock64();
     int st = 0;
     for (unsigned it = 1;; ++it) {
       if (qf_ld(f) >= a.epoch) break;
+      if ((it & 31) == 0) {
+        if (qf_ld(&a.sync[1]) == a.epoch) {
+          st = 1;
+          break;
+        }
+        if (wall_clock64() - t0 > QF_WAIT_TICKS) {
+          qf_st(&a.sync[1], a.epoch);
+          atomicAdd(a.timeouts, 1ull);
+          st = 1;
+          break;
+        }
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *sStop = st;
+  }
+  __syncthreads();
+  return *sStop == 0;
+}
+
+// Wait (all threads) until count word w reaches target.  Returns false when
+// the launch is aborted.
+static __device__ __forceinline__ bool qf_wait_count(const qf_args &a, int w, unsigned long long target, int *sStop) {
+  if (threadIdx.x == 0) {
+    unsigned long long *f = &a.sync[w];
+    const unsigned long long t0 = wall_clock64();
+    int st = 0;
+    for (unsigned it = 1;; ++it) {
+      if (qf_ld(f) >= target) break;
       if ((it & 31) == 0) {
         if (qf_ld(&a.sync[1]) == a.epoch) {
           st = 1;
@@ -123,7 +152,7 @@ static __device__ __forceinline__ void qf_scale_col(zd *col, int r0, int m, zd s
 //   sA -= V_p W2  (16-row tiles over all waves).
 // V_p must be published; T_p is waited for (flag word tflag, none if < 0)
 // after the first product.  Returns false when the launch is aborted.
-static __device__ bool qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
+static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
                                 zd *sTp, int p, int ncol, int trans, int tflag, int *sStop) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -285,7 +314,7 @@ static __device__ bool qf_apply(const qf_args &a, zd *sA, double *sPart, double 
 // into the reflector only at the next step, once no wave reads it.  Rows go
 // in batches of eight per lane, every load of a batch issued before its
 // arithmetic.
-static __device__ void qf_factor_lds(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
+static __device__ __forceinline__ void qf_factor_lds(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lm = m | 1;
   zd sck = zmk(0.0, 0.0);  // the pivot wave's 1 / (alpha - beta), kept for that store
   for (int j = 0; j < nbp; ++j) {
@@ -426,7 +455,7 @@ static __device__ __forceinline__ void qf_reflector(zd (&col)[8], zd *scol, int 
 // is done and leaves v (already scaled) in its LDS column with tau in the
 // panel's scalars; at a step the other waves then only form v^H a_c (one
 // two-value wave sum) and update their registers.
-static __device__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
+static __device__ __forceinline__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, zd *sBeta, double *sVn2) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lm = m | 1;
   const bool own = wv < nbp;
   zd col[8];
@@ -484,32 +513,32 @@ static __device__ void qf_factor_regs(zd *sA, int m, int j0, int nbp, zd *sTau, 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
-  extern __shared__ zd qf_lds[];  // zd: 16-byte aligned after the static arrays
-  __shared__ double sW2r[16][17], sW2i[16][17];
-  __shared__ zd sTau[16], sBeta[16];
-  __shared__ double sVn2[16];
-  __shared__ zd sTp[256];
-  __shared__ int sIdx, sStop;
+// Shared memory of one panel workgroup (the kernels declare it once).
+struct qf_smem {
+  double (*sW2r)[17], (*sW2i)[17];
+  zd *sTau, *sBeta;
+  double *sVn2;
+  zd *sTp;
+  int *sStop;
+};
+
+// Panel b of the QR a: steps 1-4 of the one-launch form.  With ycnt >= 0 the
+// panel's columns are waited for first: count word ycnt + b of a.sync must
+// reach ytarget (the passive workgroups of a fused launch).
+static __device__ __forceinline__ void qf_panel(const qf_args &a, int b, zd *qf_lds, const qf_smem &sm, int ycnt,
+                                                unsigned long long ytarget) {
+  double(*sW2r)[17] = sm.sW2r;
+  double(*sW2i)[17] = sm.sW2i;
+  zd *sTau = sm.sTau, *sBeta = sm.sBeta, *sTp = sm.sTp;
+  double *sVn2 = sm.sVn2;
+  int &sStop = *sm.sStop;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m = a.m, nb = a.nb;
   const int lm = m | 1;  // LDS column stride: odd, so the 16 columns of an MFMA operand hit 16 bank groups
   zd *sA = qf_lds;
   double *sPart = reinterpret_cast<double *>(qf_lds + (size_t)lm * nb);
-  if (tid == 0) {
-    const unsigned long long t =
-        __hip_atomic_fetch_add(&a.sync[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - a.ticket_base;
-    sIdx = t < (unsigned long long)a.np ? (int)t : -1;
-    sStop = 0;
-    if (sIdx < 0) {  // ticket count out of step with the host: abort the launch
-      qf_st(&a.sync[1], a.epoch);
-      atomicAdd(a.timeouts, 1ull);
-    }
-  }
-  __syncthreads();
-  const int b = sIdx;
-  if (b < 0) return;
   const int j0 = b * nb, nbp = min(nb, a.n - j0);
+  if (ycnt >= 0 && !qf_wait_count(a, ycnt + b, ytarget, &sStop)) return;
   // the panel's columns, rows [0, m); columns past the matrix are zero
   for (int e = tid; e < m * nb; e += QF_THREADS) {
     const int r = e % m, c = e / m;
@@ -657,6 +686,91 @@ __global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
   }
 }
 
+// Ticket of a launch's workgroup: 0..count-1, or -1 (launch aborted) when
+// the ticket counter is out of step with the host.
+static __device__ __forceinline__ int qf_ticket(const qf_args &a, int count, int *sIdx, int *sStop) {
+  if (threadIdx.x == 0) {
+    const unsigned long long t =
+        __hip_atomic_fetch_add(&a.sync[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - a.ticket_base;
+    *sIdx = t < (unsigned long long)count ? (int)t : -1;
+    *sStop = 0;
+    if (*sIdx < 0) {
+      qf_st(&a.sync[1], a.epoch);
+      atomicAdd(a.timeouts, 1ull);
+    }
+  }
+  __syncthreads();
+  return *sIdx;
+}
+
+__global__ __launch_bounds__(QF_THREADS) void k_qr_flow(qf_args a) {
+  extern __shared__ zd qf_lds[];  // zd: 16-byte aligned after the static arrays
+  __shared__ double sW2r[16][17], sW2i[16][17];
+  __shared__ zd sTau[16], sBeta[16];
+  __shared__ double sVn2[16];
+  __shared__ zd sTp[256];
+  __shared__ int sIdx, sStop;
+  const qf_smem sm{sW2r, sW2i, sTau, sBeta, sVn2, sTp, &sStop};
+  const int b = qf_ticket(a, a.np, &sIdx, &sStop);
+  if (b < 0) return;
+  qf_panel(a, b, qf_lds, sm, -1, 0ull);
+}
+
+// Passive workgroup c of a fused launch: columns [c nb, c nb + nb) of S get
+// H_0^H .. H_{np-1}^H of the first QR; after H_p^H, rows [p nb, p nb + nb) of
+// Q_full^H S are final and go, conjugated, into Y's columns [p nb, p nb + nb)
+// at rows [c nb, c nb + nb); then count word QF_YCNT + p of the second QR.
+static __device__ __forceinline__ void qf_passive(const qf_pair_args &P, int c, zd *qf_lds, const qf_smem &sm) {
+  const qf_args &a = P.a1;
+  double(*sW2r)[17] = sm.sW2r;
+  double(*sW2i)[17] = sm.sW2i;
+  const int tid = threadIdx.x;
+  const int m = a.m, n = a.n, nb = a.nb, lm = m | 1;
+  zd *sA = qf_lds;
+  double *sPart = reinterpret_cast<double *>(qf_lds + (size_t)lm * nb);
+  const int c0 = c * nb, ncol = min(nb, n - c0);
+  for (int e = tid; e < m * nb; e += QF_THREADS) {
+    const int r = e % m, cc = e / m;
+    sA[r + lm * cc] = cc < ncol ? P.S[r + (size_t)m * (c0 + cc)] : zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  zd *Y = P.a2.A;
+  for (int p = 0; p < a.np; ++p) {
+    if (!qf_wait(a, 2 + p, sm.sStop)) return;
+    if (!qf_apply(a, sA, sPart, sW2r, sW2i, sm.sTp, p, ncol, 1, QF_TFLAG + p, sm.sStop)) return;
+    const int r0 = p * nb, npw = min(nb, n - r0);
+    for (int e = tid; e < npw * ncol; e += QF_THREADS) {
+      const int r = e % npw, cc = e / npw;
+      qf_st_z(Y + (c0 + cc) + (size_t)P.a2.lda * (r0 + r), zconj(sA[r0 + r + lm * cc]));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
+    __syncthreads();
+    if (tid == 0) atomicAdd(&P.a2.sync[QF_YCNT + p], 1ull);
+  }
+}
+
+__global__ __launch_bounds__(QF_THREADS) void k_qr_pair(qf_pair_args P) {
+  extern __shared__ zd qf_lds[];
+  __shared__ double sW2r[16][17], sW2i[16][17];
+  __shared__ zd sTau[16], sBeta[16];
+  __shared__ double sVn2[16];
+  __shared__ zd sTp[256];
+  __shared__ int sIdx, sStop;
+  const qf_smem sm{sW2r, sW2i, sTau, sBeta, sVn2, sTp, &sStop};
+  // tickets: the first QR's panels, the passive workgroups, the second QR's
+  // panels (each waits only on lower tickets)
+  const int np1 = P.a1.np, np2 = P.a2.np;
+  const int t = qf_ticket(P.a1, 2 * np1 + np2, &sIdx, &sStop);
+  if (t < 0) return;
+  if (t < np1)
+    qf_panel(P.a1, t, qf_lds, sm, -1, 0ull);
+  else if (t < 2 * np1)
+    qf_passive(P, t - np1, qf_lds, sm);
+  else
+    qf_panel(P.a2, t - 2 * np1, qf_lds, sm, QF_YCNT, P.ytarget);
+}
+
 extern "C" size_t pqmc_qr_flow_lds(int m, int nb, int *npart) {
   const size_t base = sizeof(zd) * (size_t)(m | 1) * nb;
   const size_t budget = 160 * 1024 - 10 * 1024;  // less the kernel's static LDS
@@ -687,5 +801,32 @@ extern "C" hipError_t pqmc_qr_flow(qf_args *a, hipStream_t s) {
     }
   }
   hipLaunchKernelGGL(k_qr_flow, dim3(a->np), dim3(QF_THREADS), lds, s, *a);
+  return hipGetLastError();
+}
+
+// Launch of the fused pair: 2 np1 + np2 workgroups; the tickets come from
+// a1's counter (ticket_base there), the epochs are each QR's own.
+extern "C" hipError_t pqmc_qr_pair(qf_pair_args *P, hipStream_t s) {
+  qf_args &a1 = P->a1, &a2 = P->a2;
+  if (a1.m > 1024 || a1.n > a1.m || a1.n <= 0 || (a1.nb != 8 && a1.nb != 16) || a1.np != (a1.n + a1.nb - 1) / a1.nb ||
+      a1.np > 128 || a2.m != a1.n || a2.n != a1.n || a2.nb != a1.nb || a2.np != a1.np || !P->S || a2.lda != a1.n)
+    return hipErrorInvalidValue;
+  int npart1 = 0, npart2 = 0;
+  const size_t lds1 = pqmc_qr_flow_lds(a1.m, a1.nb, &npart1), lds2 = pqmc_qr_flow_lds(a2.m, a2.nb, &npart2);
+  const size_t lds = lds1 > lds2 ? lds1 : lds2;
+  if (lds > 160 * 1024 - 10 * 1024) return hipErrorInvalidValue;
+  a1.npart = npart1;
+  a2.npart = npart2;
+  static std::mutex attr_mu;
+  static size_t attr = 0;
+  {
+    std::lock_guard<std::mutex> lk(attr_mu);
+    if (lds > attr) {
+      hipError_t e = hipFuncSetAttribute((const void *)k_qr_pair, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = lds;
+    }
+  }
+  hipLaunchKernelGGL(k_qr_pair, dim3(2 * a1.np + a2.np), dim3(QF_THREADS), lds, s, *P);
   return hipGetLastError();
 }

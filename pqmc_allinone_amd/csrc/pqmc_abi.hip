@@ -48,7 +48,7 @@ hipError_t pqmc_qdr_finish(const zd *A, int lda, int n, zd *Rout, zd *D, int con
 hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, zd *work, hipStream_t s);
 hipError_t pqmc_lncw(int ne, const zd *betaM, const zd *tauM, const double *vn2M, const zd *dlog, int nblk,
                      int tr_weight, int ncopy, zd *out, const double *acc, unsigned long long *drift_call,
-                     unsigned long long *drift_run, hipStream_t s);
+                     unsigned long long *drift_run, int conjq, hipStream_t s);
 hipError_t pqmc_dlog(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_dlog_conj(const zd *D, int n, zd *out, hipStream_t s);
 hipError_t pqmc_maxdiff(const zd *a, const zd *b, long n, unsigned long long *out, unsigned long long *run,
@@ -93,6 +93,7 @@ struct qr_ws {  // workspace of one blocked QR
   // counts launches (epoch) and the tickets they took
   unsigned long long *sync = nullptr;
   unsigned long long epoch = 0, tickets = 0;
+  unsigned long long ycount = 0;  // the fused launch's column counts so far (per panel)
 };
 
 struct ev_pool {
@@ -520,6 +521,78 @@ static int run_qr(pqmc_handle *h, qr_ws &w, zd *A, int lda, int m, int n, zd *Q,
   return PQMC_OK;
 }
 
+// The stabilization's two QRs in one launch (k_qr_pair): the fresh string W
+// (Ns x ne) with its explicit Q into Qout, and beside it the stored string S,
+// so that Y = (Q_full^H S)[0:ne, :]^H = S^H Qout lands in d_M column block by
+// column block and is factored there (Q_Y in d_QM, R_Y in d_M) as it lands.
+// Forward (fresh right string, S the left one) Y = QL^H QR = M; backward
+// (fresh left string, S the right one) Y = M^H.
+static int qr_pair(pqmc_handle *h, zd *W, zd *Qout, const zd *S) {
+  const int Ns = h->Ns, ne = h->ne, nb = h->nbq, np = (ne + nb - 1) / nb;
+  qf_pair_args P{};
+  qf_args &a1 = P.a1, &a2 = P.a2;
+  a1.A = W;
+  a1.lda = Ns;
+  a1.m = Ns;
+  a1.n = ne;
+  a1.nb = nb;
+  a1.np = np;
+  a1.Q = Qout;
+  a1.V = h->qa.V;
+  a1.T = h->qa.T;
+  a1.tau = h->qa.tau;
+  a1.beta = h->qa.beta;
+  a1.vn2 = h->qa.vn2;
+  a1.sync = h->qa.sync;
+  a1.timeouts = h->d_stats + 2;
+  a1.epoch = h->qa.epoch + 1;
+  a1.ticket_base = h->qa.tickets;
+  a2 = a1;
+  a2.A = h->d_M;
+  a2.lda = ne;
+  a2.m = ne;
+  a2.Q = h->d_QM;
+  a2.V = h->qm.V;
+  a2.T = h->qm.T;
+  a2.tau = h->qm.tau;
+  a2.beta = h->qm.beta;
+  a2.vn2 = h->qm.vn2;
+  a2.sync = h->qm.sync;
+  a2.epoch = h->qm.epoch + 1;
+  a2.ticket_base = 0;  // the tickets come from a1's counter
+  P.S = S;
+  P.ytarget = h->qm.ycount + np;
+  HIPCHK(pqmc_qr_pair(&P, h->st));
+  h->qa.epoch += 1;
+  h->qa.tickets += 3 * (unsigned long long)np;
+  h->qm.epoch += 1;
+  h->qm.ycount += np;
+  return PQMC_OK;
+}
+
+// The rest of G after qr_pair: with QLt, QRt the strings such that Y =
+// QLt^H QRt (the left and right strings forward, the right and left ones
+// backward), Gt = QRt Y^-1 QLt^H is G forward and G^H backward, and
+// ln det Y is ln det M, or its conjugate (cal_g_and_det_T0,
+// src/mod_evolution.f90:330-357).
+static int cal_g_pair(pqmc_handle *h, const zd *QLt, const zd *QRt, zd *Gout, bool backward) {
+  const int Ns = h->Ns, ne = h->ne;
+  const zd one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // Z = Q_Y^H QLt^H  (ne x Ns), then R_Y^-1 Z
+  HIPCHK(pqmc_zgemm(1, 1, ne, Ns, ne, one, h->d_QM, ne, QLt, Ns, zero, h->d_Z, ne, h->st));
+  HIPCHK(pqmc_trsm_upper(h->d_M, ne, ne, h->d_Z, ne, Ns, h->d_rinv, h->st));
+  if (backward) {
+    HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRt, Ns, h->d_Z, ne, zero, h->d_scr, Ns, h->st));
+    HIPCHK(pqmc_conjt(h->d_scr, Ns, Ns, Ns, Gout, Ns, h->st));
+  } else {
+    HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRt, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
+  }
+  HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_dlog, h->nblock, h->md.tr_weight, h->md.ncopy,
+                   h->d_lncw, h->d_lncw_acc, h->d_err + 1, h->d_err + 3, backward ? 1 : 0, h->st));
+  HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
+  return PQMC_OK;
+}
+
 // G = QR (QL^H QR)^-1 QL^H with ln_cw (cal_g_and_det_T0, src/mod_evolution.f90:330-357).
 // QL is the left B-string stored as lmat^H (Ns x ne); QR is the right one.
 // drift: a stabilization (get_g_scratch), whose ln_cw is compared with the
@@ -538,7 +611,7 @@ static int cal_g(pqmc_handle *h, const zd *QL, const zd *QRm, zd *Gout, bool dri
   // G = QR Z
   HIPCHK(pqmc_zgemm(0, 0, Ns, Ns, ne, one, QRm, Ns, h->d_Z, ne, zero, Gout, Ns, h->st));
   HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_dlog, h->nblock, h->md.tr_weight, h->md.ncopy,
-                   h->d_lncw, h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, h->st));
+                   h->d_lncw, h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, 0, h->st));
   // the accepts' log(real(R_elec)) terms since the last stabilization are
   // overwritten with it (src/mod_update.f90:142)
   HIPCHK(hipMemsetAsync(h->d_lncw_acc, 0, sizeof(double), h->st));
@@ -734,7 +807,7 @@ static int ft_weight(pqmc_handle *h, ft_udt r, ft_udt l, zd *QM, zd *lncw, bool 
   int rc = run_qr(h, h->qf, M, Ns, Ns, Ns, QM, nullptr, nullptr, 0);
   if (rc) return rc;
   HIPCHK(pqmc_lncw(Ns, h->qf.beta, h->qf.tau, h->qf.vn2, h->d_flog, 4, h->md.tr_weight, h->md.ncopy, lncw,
-                   h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, h->st));
+                   h->d_lncw_acc, drift ? h->d_err + 1 : nullptr, h->d_err + 3, 0, h->st));
   return PQMC_OK;
 }
 
@@ -976,6 +1049,7 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
   HIPCHK(hipMemsetAsync(h->d_err, 0, 2 * sizeof(unsigned long long), h->st));
   const size_t qb = sizeof(zd) * Ns * ne;
   const zd *lmatH = nullptr, *rmat = nullptr;
+  bool paired = false;  // the two QRs ran as one launch (the default one-launch QR mode)
   if (h->ft) {
     rc = ft_scratch(h, time, forward);
     if (rc) return rc;
@@ -985,11 +1059,21 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block - 1), qb, hipMemcpyDeviceToDevice, h->st));
     rc = string_step(h, h->d_W, time - ng, ng, 0);
     if (rc) return rc;
-    rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
-    if (rc) return rc;
-    HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
     lmatH = QS(h, block + 1);
     rmat = QS(h, block);
+    if (h->qr_mode == 0) {
+      rc = qr_pair(h, h->d_W, QS(h, block), lmatH);
+      if (rc) return rc;
+      HIPCHK(pqmc_qdr_finish(h->d_W, Ns, ne, RS(h, block), DS(h, block), 0, h->st));
+      HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
+      rc = cal_g_pair(h, lmatH, rmat, h->d_Gn, false);
+      if (rc) return rc;
+      paired = true;
+    } else {
+      rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 0);
+      if (rc) return rc;
+      HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
+    }
   } else {
     int block = time / ng + 2;
     if (ng == 1) block -= 1;
@@ -997,13 +1081,23 @@ static int g_scratch(pqmc_handle *h, int time, int forward) {
     HIPCHK(hipMemcpyAsync(h->d_W, QS(h, block + 1), qb, hipMemcpyDeviceToDevice, h->st));
     rc = string_step(h, h->d_W, time + ng - 1, ng, 1);
     if (rc) return rc;
-    rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 1);
-    if (rc) return rc;
-    HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
     lmatH = QS(h, block);
     rmat = QS(h, block - 1);
+    if (h->qr_mode == 0) {
+      rc = qr_pair(h, h->d_W, QS(h, block), rmat);
+      if (rc) return rc;
+      HIPCHK(pqmc_qdr_finish(h->d_W, Ns, ne, RS(h, block), DS(h, block), 1, h->st));
+      HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
+      rc = cal_g_pair(h, rmat, lmatH, h->d_Gn, true);
+      if (rc) return rc;
+      paired = true;
+    } else {
+      rc = run_qr(h, h->qa, h->d_W, Ns, Ns, ne, QS(h, block), DS(h, block), RS(h, block), 1);
+      if (rc) return rc;
+      HIPCHK(pqmc_dlog(DS(h, block), ne, h->d_dlog + block - 1, h->st));
+    }
   }
-  if (!h->ft) {
+  if (!h->ft && !paired) {
     rc = cal_g(h, lmatH, rmat, h->d_Gn, true);
     if (rc) return rc;
   }
@@ -1463,7 +1557,7 @@ extern "C" int pqmc_global_weight(pqmc_handle *h, const double *x, double *ri) {
   rc = run_qr(h, h->qm, h->d_M, ne, ne, ne, nullptr, nullptr, nullptr, 0);
   if (rc) return rc;
   HIPCHK(pqmc_lncw(ne, h->qm.beta, h->qm.tau, h->qm.vn2, h->d_glog, nlog, h->md.tr_weight, h->md.ncopy, h->d_gw,
-                   nullptr, nullptr, nullptr, h->st));
+                   nullptr, nullptr, nullptr, 0, h->st));
   zd v;
   HIPCHK(hipMemcpyAsync(&v, h->d_gw, sizeof(zd), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
