@@ -388,6 +388,49 @@ def kernel_classes(reps):
     print(json.dumps({"kernel_classes": reps}))
 
 
+def pass_phases():
+    """Phase timers of the family pass (pqmc_update_profile) at 14x14,
+    Lt=1000: one warm-up sweep, then one forward and one backward sweep with
+    the timers on; per role the wall-clock microseconds per pass in each of
+    the kernel's eight timer slots (their meaning per role is in
+    k_pass.hip's PROF_ACC marks)."""
+    import ctypes as C
+    from pqmc_allinone_amd import _lib
+    from pqmc_allinone_amd.host import Chain, Ran2
+    from pqmc_allinone_amd.model import ModelParams, build_model, synthetic_inputs
+    L, Lt, ng, ne, seed = CONFIGS["c2"]
+    m = build_model(ModelParams(L1=L, L2=L, Lt=Lt, ngroup=ng, ne=ne, lam=0.64, omega=1.0))
+    x, sl = synthetic_inputs(m, seed)
+    ch = Chain(m)
+    ch.set_fields(x)
+    ch.set_trial(sl)
+    ch.init_g(1)
+    rng = Ran2(seed)
+    nd = 2 * m.Nb * Lt
+    ch.sweep(False, rng.draws(nd), measure=True)
+    lib = _lib.lib()
+    P = C.c_int32(0)
+    out = {}
+    for fwd in (True, False):
+        _lib.check(lib.pqmc_update_profile(ch.h, 1, None, C.addressof(P)))
+        ch.timing(0, True)
+        ch.sweep(fwd, rng.draws(nd), measure=True)
+        ch.synchronize()
+        ms, n = ch.timing(0, False)
+        G = P.value
+        prof = np.zeros(8 * G)
+        _lib.check(lib.pqmc_update_profile(ch.h, 0, prof.ctypes.data, C.addressof(P)))
+        prof = prof.reshape(G, 8) / n
+        H = G - 5
+        rows = {"leader wave 0": prof[0], "helpers (mean)": prof[1:H + 1].mean(0), "leader wave 1": prof[H + 1],
+                "leader wave 2": prof[H + 2], "leader wave 6": prof[H + 3], "leader wave 7": prof[H + 4]}
+        out["forward" if fwd else "backward"] = {"us_per_pass": 1e3 * ms / n,
+                                                 **{k: [round(float(v), 2) for v in r] for k, r in rows.items()}}
+        log(f"pass phases, {'forward' if fwd else 'backward'} sweep: {1e3 * ms / n:.1f} us per pass")
+    ch.close()
+    print(json.dumps({"pass_phases": out}))
+
+
 def copy_calibration(mib):
     """Elementwise copy kernel over `mib` MiB of f64: FETCH_SIZE and WRITE_SIZE of
     one dispatch should each read mib * 1024 KiB; the ratio scales the counters
@@ -433,6 +476,8 @@ def main():
     ap.add_argument("--mixed-sweeps", type=int, default=2, help="timed sweeps of the mixed batch")
     ap.add_argument("--kernel-classes", type=int, default=0, metavar="REPS",
                     help="only run REPS of each kernel class other than the family pass (counter collection)")
+    ap.add_argument("--pass-phases", action="store_true",
+                    help="only print the family pass's phase timers at 14x14, Lt=1000 (one sweep each way)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size reference for the HBM byte counters")
@@ -450,6 +495,9 @@ def main():
         return
     if args.kernel_classes:
         kernel_classes(args.kernel_classes)
+        return
+    if args.pass_phases:
+        pass_phases()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); chains shard across GPUs with no "

@@ -30,7 +30,7 @@
 struct __align__(128) ps_rec {
   int li[PS_NB];          // block-local indices of the accepted bonds, ascending
   int pad[32 - PS_NB];
-  zd AT[PS_BS * PS_BS];   // X on the diagonal blocks, X V above, T below; row-major, stride PS_BS
+  zd AT[PS_BS * PS_BS];   // X on the diagonal blocks, V above, T below; row-major, stride PS_BS
 };
 static_assert(sizeof(ps_rec) % 128 == 0, "a decision record must be a whole number of 128-byte lines");
 static_assert(offsetof(ps_rec, AT) == 128, "the record header must fill exactly one 128-byte line");

@@ -12,9 +12,13 @@
 // stretch's start G:
 //     G_after[:, q] = G[:, q] + sum_m G[:, S_m] y_m(q),
 //     v_m(q) = w_m(q) - sum_{m'<m} T[m][m'] v_m'(q),   w_m(q) = (I - G)[S_m, q],
-//     y_m(q) = X_m v_m(q) + sum_{m'>m} (X_m V[m][m']) y_m'(q)   (m = a-1 .. 0),
+//     y_m(q) = X_m (v_m(q) + sum_{m'>m} V[m][m'] y_m'(q))   (m = a-1 .. 0),
 // with the 4 x 4 blocks T[m][m'] = u_m'[S_m] and V[m'][m] = v_m'[S_m]
-// (forward and backward block substitutions).  The bonds are cut into
+// (forward and backward block substitutions).  T holds X as the reference's
+// u does (G[:, S] X is formed first, src/mod_update.f90:177); in y the sum
+// is formed before X_m multiplies it, which keeps the drift after an
+// ill-conditioned accept (large X_m) at the level of the sequential
+// updates.  The bonds are cut into
 // decision blocks of nb consecutive bonds; one leader workgroup decides them
 // and H helper workgroups own the columns of G:
 //
@@ -22,7 +26,7 @@
 //    W = G[S_k, S_k] that it keeps current with the rank-4 updates (only the
 //    rows and columns of the block's later bonds change); wave 6 takes the
 //    part of each update off W's diagonal 4 x 4 blocks.
-//  * Leader, wave 1: copies each accept's X_m, T and X V blocks (from wave
+//  * Leader, wave 1: copies each accept's X_m, T and V blocks (from wave
 //    0's u_m, v_m at the block's sites) to the block's record and publishes
 //    it as soon as the block is decided.
 //  * Leader, waves 2-5: fetch G[S_{k+1}, S_k], G[S_k, S_{k+1}] and
@@ -723,7 +727,7 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
       PROF_ACC(0);
     } else if (wave == 1) {
       // ---- the record of block k, accept by accept: wave 0's 4 x 4 blocks
-      // X_m, T[m][m'] = u_m'[S_m] and X_m' V[m'][m] = X_m' v_m'[S_m] (m' < m),
+      // X_m, T[m][m'] = u_m'[S_m] and V[m'][m] = v_m'[S_m] (m' < m),
       // which are final once accept m is handed over ----
       int md_ = 0;
       ps_rec *R = A.rec + k;
@@ -744,10 +748,7 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
               st_coh(&R->AT[(4 * m + r) * PS_BS + 4 * mp + c], L.sU[mp][(j4 + r) * 4 + c]);
             } else {
               const int mp = blk - 1 - m;
-              zd xv = zmk(0.0, 0.0);
-#pragma unroll
-              for (int e2 = 0; e2 < 4; ++e2) xv = zadd(xv, zmul(L.sX[mp][r + 4 * e2], L.sV[mp][e2 * PS_BS + j4 + c]));
-              st_coh(&R->AT[(4 * mp + r) * PS_BS + 4 * m + c], xv);
+              st_coh(&R->AT[(4 * mp + r) * PS_BS + 4 * m + c], L.sV[mp][r * PS_BS + j4 + c]);
             }
           }
         }
@@ -966,9 +967,9 @@ static __device__ __forceinline__ void ps_announce(const ps_args &A, int b0, int
 }
 
 // y[q] = (y_0, .., y_{a-1}) of the own columns q from the record (the 4 x 4
-// blocks X, T and X V of ps_rec):
+// blocks X, T and V of ps_rec):
 //     v_m = w_m - sum_{m'<m} T[m][m'] v_m',    w_m = (I - G)[S_m, q],
-//     y_m = X_m v_m + sum_{m'>m} (X_m V[m][m']) y_m'   (m = a-1 .. 0),
+//     y_m = X_m z_m,  z_m = v_m + sum_{m'>m} V[m][m'] y_m'   (m = a-1 .. 0),
 // so that G[:, q] += G[:, S_A] y[q] is the block's rank-4 updates in turn.
 // Wave w takes columns 4w..4w+3; lane (ql, c, d) = (l / 16, l / 4 % 4, l % 4)
 // forms component c of column 4w+ql, its sums over d reduced in the quad.
@@ -1001,20 +1002,25 @@ static __device__ __forceinline__ void ps_helper_y(const ps_args &A, const ps_he
     if (lead) sv[q * PS_BS + 4 * m + c] = acc;
     wave_sync();
   }
-  // y_m = X_m v_m + sum_{m'>m} (X_m V[m][m']) y_m'  (the record holds X_m V above the diagonal)
+  // z_m = v_m + sum_{m'>m} V[m][m'] y_m', then y_m = X_m z_m (the record holds V above the diagonal)
   for (int m = a - 1; m >= 0; --m) {
     zd acc0 = zmk(0.0, 0.0), acc1 = acc0;
     if (qok) {
-      acc0 = zmul(M[(4 * m + c) * PS_BS + 4 * m + d], sv[q * PS_BS + 4 * m + d]);
+      if (lead) acc0 = sv[q * PS_BS + 4 * m + c];
       for (int mp = m + 1; mp < a; mp += 2) {
         acc1 = zadd(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], sy[q * PS_BS + 4 * mp + d]));
         if (mp + 1 < a) acc0 = zadd(acc0, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], sy[q * PS_BS + 4 * mp + 4 + d]));
       }
     }
-    zd yv = zadd(acc0, acc1);
-    yv.x = quad_sum(yv.x);
-    yv.y = quad_sum(yv.y);
-    if (lead) sy[q * PS_BS + 4 * m + c] = yv;
+    const zd xdc = M[(4 * m + d) * PS_BS + 4 * m + c];  // X_m[d][c]
+    zd zv = zadd(acc0, acc1);
+    zv.x = quad_sum(zv.x);
+    zv.y = quad_sum(zv.y);
+    // lane (ql, c, d) holds z_m[c]; y_m[d] = sum_c X_m[d][c] z_m[c] over the row's lanes of equal d
+    zd yv = zmul(xdc, zv);
+    yv.x = row4_sum(yv.x);
+    yv.y = row4_sum(yv.y);
+    if (qok && c == 0) sy[q * PS_BS + 4 * m + d] = yv;
     wave_sync();
   }
 }
