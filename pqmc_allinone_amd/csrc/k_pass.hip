@@ -537,7 +537,11 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
   // a backward pass wraps every column with the exponentials before the
   // update; none may change until all helpers have read them
   bool ekv_free = A.wrap != 2;
-  // proposals of block 0; the window after the helpers' load
+  // Proposals of block 0 (wave 0) while the other waves gather block 0's
+  // window from G itself, W = G[S_0, S_0]: E is block diagonal over the
+  // bonds, so on a backward pass the helpers' wrap E^-1 G E restricted to
+  // S_0 is E^-1[S_0] G[S_0, S_0] E[S_0], formed here with their operations
+  // in their order (columns, then rows), which gives their values.
   if (tid < min(nb, npla)) ps_propose(A, tid, L.prop[0][tid]);
   if (tid == 0) {
     L.abort = 0;
@@ -545,15 +549,42 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
     L.done = 0;
     L.fbar = 0;
     L.wdone = 0;
-    L.nacc = ps_wait(A, ps_pub(A, -1, 0), ps_owners(A, 0), nullptr, 0) ? 1 : 0;
+    L.nacc = 1;
   }
-  __syncthreads();
-  ok = L.nacc != 0;
-  if (ok) {
-    const int n0 = ps_nsites(A, 0);
-    for (int e = tid; e < PS_BS * PS_BS; e += PS_THREADS) {
-      const int c = e >> 5, r = e & 31;
-      L.buf[iW][r * PS_BS + c] = zsel(r < n0 && c < n0, ps_tcol(A, -1, 0, c)[r]);
+  {
+    const int n0 = ps_nsites(A, 0), Ns = md.Ns;
+    zd *W = L.buf[iW];
+    for (int e = tid - 64; e >= 0 && e < PS_BS * PS_BS; e += PS_THREADS - 64) {
+      const int r = e >> 5, c = e & 31;
+      W[r * PS_BS + c] = (r < n0 && c < n0) ? A.G[A.fsites[r] + (size_t)Ns * A.fsites[c]] : zmk(0.0, 0.0);
+    }
+    __syncthreads();
+    if (A.wrap == 2) {
+      const ekv_t *ek = A.ekv + ((size_t)(A.t - 1) * 4 + A.f) * npla;
+      const int nb0 = n0 / 4;
+      for (int e = tid; e < nb0 * n0; e += PS_THREADS) {  // columns: G E
+        const int bl = e / n0, r = e % n0;
+        ekv_t E;
+        ld_coh_ekv(ek + bl, E);
+        zd v[4], y[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = W[r * PS_BS + 4 * bl + c];
+        ekv_apply_right(E, false, v, y);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) W[r * PS_BS + 4 * bl + c] = y[c];
+      }
+      __syncthreads();
+      for (int e = tid; e < nb0 * n0; e += PS_THREADS) {  // rows: E^-1 G
+        const int bb = e / n0, q = e % n0;
+        ekv_t E;
+        ld_coh_ekv(ek + bb, E);
+        zd v[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) v[a] = W[(4 * bb + a) * PS_BS + q];
+        ekv_apply(E, true, v, y);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) W[(4 * bb + a) * PS_BS + q] = y[a];
+      }
     }
   }
   __syncthreads();
