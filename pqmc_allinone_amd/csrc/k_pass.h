@@ -35,6 +35,14 @@ struct __align__(128) ps_rec {
 static_assert(sizeof(ps_rec) % 128 == 0, "a decision record must be a whole number of 128-byte lines");
 static_assert(offsetof(ps_rec, AT) == 128, "the record header must fill exactly one 128-byte line");
 
+// Proposal of one bond for the leader's decisions (ps_propose).
+struct ps_prop {
+  zd D[16];  // Delta = E(old)^-1 E(new) - I, column-major
+  ekv_t en;
+  double thr, xnew;
+  int fld, pad[3];
+};
+
 struct ps_args {
   model_t md;
   int f, t, wrap;    // family (0-based), slice (1-based); wrap 0 none, 1 E G E^-1 after, 2 E^-1 G E before
@@ -55,6 +63,17 @@ struct ps_args {
                              // published their tile-slot columns after step k (k = -1..nblk-1),
                              // [10 + 2 nblk + k] block k decided: accepts + 1, [10 + 3 nblk + k + 1]
                              // owners of block k+1 that have published their slot columns after step k
+  // block 0's proposals formed by the previous pass (or null: formed here);
+  // this pass's wave 7 forms those of the next pass (family nx_f, slice nx_t,
+  // its draws and tables) into nxt, unless null.  The next pass reads no
+  // field or exponential that this one changes (another family, or another
+  // slice with another family).
+  const ps_prop *pre;
+  ps_prop *nxt;
+  int nx_f, nx_t;
+  const float *nx_u;
+  const int *nx_fbfi, *nx_fcross;
+  const zd *nx_fph;
   uint8_t *acc;       // [npla]
   double *lncw_acc;
   unsigned long long *stats;  // [0] accepted, [1] rejected, [2] wait timeouts

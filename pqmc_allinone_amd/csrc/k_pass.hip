@@ -51,13 +51,6 @@
 
 #include "k_pass.h"
 
-struct ps_prop {
-  zd D[16];  // Delta = E(old)^-1 E(new) - I, column-major
-  ekv_t en;
-  double thr, xnew;
-  int fld, pad[3];
-};
-
 // Data exchanged between the workgroups of a pass (records, slot columns,
 // updated exponentials) is written only with device-scope (sc1) stores,
 // which go through to memory, and a publisher waits for its own stores to
@@ -288,16 +281,19 @@ static __device__ __forceinline__ zd minor3(const zd *M, int ri, int cj) {
 // Proposal of bond p (family order): jump, bosonic action change
 // (cal_energy_difference, :203-228), threshold, Delta (calculate_Delta,
 // :230-245) and the new compact exponential.
-static __device__ __forceinline__ void ps_propose(const ps_args &A, int p, ps_prop &out) {
+// Family f at slice t with its draws u and tables (A.f, A.t, A.u, ... for
+// this pass).
+static __device__ __forceinline__ void ps_propose(const ps_args &A, int f, int t, const float *u, const int *fbfi,
+                                                  const int *fcross, const zd *fph, int p, ps_prop &out) {
   const model_t &md = A.md;
-  const size_t tbase = (size_t)md.Nb * (A.t - 1);
-  const int fld = A.fbfi[p];
+  const size_t tbase = (size_t)md.Nb * (t - 1);
+  const int fld = fbfi[p];
   const double xold = A.x[fld + tbase];
   // ran_sysm = ran2*2-1 in REAL(SP) (src/mod_ranstate.f90:233-236); jump in DP
-  const float rs = A.u[2 * p] * 2.0f - 1.0f;
+  const float rs = u[2 * p] * 2.0f - 1.0f;
   const double jump = md.jump * (double)rs;
-  const double be = A.x[fld + (size_t)md.Nb * (modi_d(A.t - 1, md.Lt) - 1)];
-  const double af = A.x[fld + (size_t)md.Nb * (modi_d(A.t + 1, md.Lt) - 1)];
+  const double be = A.x[fld + (size_t)md.Nb * (modi_d(t - 1, md.Lt) - 1)];
+  const double af = A.x[fld + (size_t)md.Nb * (modi_d(t + 1, md.Lt) - 1)];
   const double nw = xold + jump;
   const double dt2 = md.dtau * md.dtau;
   const double e1 = 0.5 * md.M * ((be - xold) * (be - xold) + (af - xold) * (af - xold)) / dt2 +
@@ -306,13 +302,13 @@ static __device__ __forceinline__ void ps_propose(const ps_args &A, int p, ps_pr
                     0.5 * md.D * (nw * nw) - md.biased * nw;
   const double dS = -md.dtau * (e2 - e1);
   const double kpow = (double)md.ncopy * (md.tr_weight ? 2.0 : 1.0);
-  out.thr = exp(2.0 * (log((double)A.u[2 * p + 1]) - dS) / kpow);
+  out.thr = exp(2.0 * (log((double)u[2 * p + 1]) - dS) / kpow);
   out.xnew = nw;
   out.fld = fld;
   ekv_t en;
-  expkv_compact(md, A.f, nw, A.fcross[p] ? &A.fph[4 * p] : nullptr, en);
+  expkv_compact(md, f, nw, fcross[p] ? &fph[4 * p] : nullptr, en);
   out.en = en;
-  const ekv_t eo = A.ekv[((size_t)(A.t - 1) * 4 + A.f) * md.npla + p];
+  const ekv_t eo = A.ekv[((size_t)(t - 1) * 4 + f) * md.npla + p];
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
@@ -542,7 +538,12 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
   // bonds, so on a backward pass the helpers' wrap E^-1 G E restricted to
   // S_0 is E^-1[S_0] G[S_0, S_0] E[S_0], formed here with their operations
   // in their order (columns, then rows), which gives their values.
-  if (tid < min(nb, npla)) ps_propose(A, tid, L.prop[0][tid]);
+  if (tid < min(nb, npla)) {
+    if (A.pre)
+      L.prop[0][tid] = A.pre[tid];
+    else
+      ps_propose(A, A.f, A.t, A.u, A.fbfi, A.fcross, A.fph, tid, L.prop[0][tid]);
+  }
   if (tid == 0) {
     L.abort = 0;
     L.npub = 0;
@@ -849,7 +850,11 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
     } else if (has_next && wave == 7) {
       // ---- wave 7: the next block's proposals ----
       const int p1 = (k + 1) * nb, nb1 = ps_nsites(A, k + 1) / 4;
-      if (lane < nb1) ps_propose(A, p1 + lane, L.prop[(k + 1) & 1][lane]);
+      if (lane < nb1) ps_propose(A, A.f, A.t, A.u, A.fbfi, A.fcross, A.fph, p1 + lane, L.prop[(k + 1) & 1][lane]);
+      PROF_ACC(1);
+    } else if (!has_next && wave == 7 && A.nxt) {
+      // ---- wave 7, last block: the next pass's proposals of its block 0 ----
+      if (lane < min(nb, npla)) ps_propose(A, A.nx_f, A.nx_t, A.nx_u, A.nx_fbfi, A.nx_fcross, A.nx_fph, lane, A.nxt[lane]);
       PROF_ACC(1);
     }
     if (tid == 0 && ps_ld(&A.sync[2]) != 0) L.abort = 1;

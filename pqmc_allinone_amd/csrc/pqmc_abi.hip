@@ -130,6 +130,7 @@ struct pqmc_handle {
   zd *d_slot = nullptr;
   zd *d_tslot = nullptr;
   ps_rec *d_rec = nullptr;
+  ps_prop *d_pre = nullptr;  // [2][PS_NB] block-0 proposals handed from one pass of a sweep to the next
   unsigned long long *d_psync = nullptr;
   double *d_lncw_acc = nullptr;
   zd *d_grad = nullptr;  // field gradient, [Lt][Nb]
@@ -317,6 +318,7 @@ extern "C" int pqmc_create(const pqmc_config *cfg, pqmc_handle **out) {
   A_(h->d_slot, pqmc_pass_slot_elems(Ns, h->nblk));
   A_(h->d_tslot, pqmc_pass_tslot_elems(h->nblk));
   A_(h->d_rec, (size_t)h->nblk);
+  A_(h->d_pre, 2 * (size_t)PS_NB);
   A_(h->d_psync, pqmc_pass_sync_words(h->nblk));
   A_(h->d_lncw_acc, 1);
   A_(h->d_lncw, 1);
@@ -388,7 +390,7 @@ extern "C" int pqmc_destroy(pqmc_handle *h) {
   void *ptrs[] = {h->d_sites, h->d_bfi, h->d_cross, h->d_pos, h->d_ph, h->d_x, h->d_ekv, h->d_G, h->d_Gn, h->d_scr,
                   h->d_Q, h->d_D, h->d_R, h->d_dlog, h->d_rinv, h->d_slQ, h->d_slD, h->d_slR, h->d_W, h->d_L, h->d_M, h->d_QM,
                   h->d_Z, h->qa.V, h->qa.T, h->qa.tau, h->qa.beta, h->qa.vn2, h->qm.V, h->qm.T, h->qm.tau,
-                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_psync, h->d_lncw_acc,
+                  h->qm.beta, h->qm.vn2, h->d_u, h->d_acc, h->d_slot, h->d_tslot, h->d_rec, h->d_pre, h->d_psync, h->d_lncw_acc,
                   h->d_lncw, h->d_stats, h->d_err, h->d_hist, h->d_meas, h->d_kphase, h->d_prof,
                   h->d_gx, h->d_gekv, h->d_glog, h->d_gw, h->d_FQ, h->d_FT, h->d_FD, h->d_FL, h->d_fw,
                   h->d_flog, h->d_fperm, h->qf.V, h->qf.T, h->qf.tau, h->qf.beta, h->qf.vn2, h->d_gft, h->d_grad,
@@ -929,7 +931,17 @@ static int ensure_u(pqmc_handle *h, size_t n) {
 // One family pass (update, optionally fused with the family's two-way wrap:
 // wrap 2 = E^-1 G E before the update, 1 = E G E^-1 after it); d_u points at
 // its 2*npla draws on the device.
-static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u, uint8_t *d_acc) {
+// A pass of a sweep may take block 0's proposals from the pass before it
+// (pre) and form those of the pass after it (next: family, slice, draws).
+struct pass_link {
+  const ps_prop *pre = nullptr;
+  ps_prop *nxt = nullptr;
+  int nf = 0, nt = 0;
+  const float *nu = nullptr;
+};
+
+static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u, uint8_t *d_acc,
+                       const pass_link &lk = pass_link()) {
   ps_args a;
   a.md = h->md;
   a.f = f;
@@ -952,6 +964,14 @@ static int launch_pass(pqmc_handle *h, int f, int t, int wrap, const float *d_u,
   a.tslot = h->d_tslot;
   a.rec = h->d_rec;
   a.sync = h->d_psync;
+  a.pre = lk.pre;
+  a.nxt = lk.nxt;
+  a.nx_f = lk.nf;
+  a.nx_t = lk.nt;
+  a.nx_u = lk.nu;
+  a.nx_fbfi = h->d_bfi + (size_t)lk.nf * h->npla;
+  a.nx_fcross = h->d_cross + (size_t)lk.nf * h->npla;
+  a.nx_fph = h->d_ph + (size_t)lk.nf * h->npla * 4;
   a.acc = d_acc;
   a.lncw_acc = h->d_lncw_acc;
   a.stats = h->d_stats;
@@ -1166,21 +1186,36 @@ extern "C" int pqmc_sweep(pqmc_handle *h, int forward, const float *u, int ntc, 
   size_t off = 0;
   bool measured = false;
   h->nhist = 0;
+  // passes in sweep order; each hands block 0's proposals of the next one on
+  int ip = 0;
+  const int npass = 4 * ntc;
+  auto pass = [&](int f, int time, int wrap) -> int {
+    pass_link lk;
+    if (ip > 0) lk.pre = h->d_pre + (size_t)(ip & 1) * PS_NB;
+    if (ip + 1 < npass) {
+      lk.nxt = h->d_pre + (size_t)((ip + 1) & 1) * PS_NB;
+      lk.nf = forward ? (f + 1) % 4 : (f + 3) % 4;
+      lk.nt = forward ? (f == 3 ? time + 1 : time) : (f == 0 ? time - 1 : time);
+      lk.nu = h->d_u + off + 2 * (size_t)npla;
+    }
+    const int r = launch_pass(h, f, time, wrap, h->d_u + off, h->d_acc + off / 2, lk);
+    off += 2 * (size_t)npla;
+    ++ip;
+    return r;
+  };
   for (int tc = 1; tc <= ntc; ++tc) {
     int time;
     if (!forward) {
       time = Lt - tc + 1;
       for (int f = 3; f >= 0; --f) {
-        rc = launch_pass(h, f, time, 2, h->d_u + off, h->d_acc + off / 2);
+        rc = pass(f, time, 2);
         if (rc) return rc;
-        off += 2 * (size_t)npla;
       }
     } else {
       time = tc;
       for (int f = 0; f < 4; ++f) {
-        rc = launch_pass(h, f, time, 1, h->d_u + off, h->d_acc + off / 2);
+        rc = pass(f, time, 1);
         if (rc) return rc;
-        off += 2 * (size_t)npla;
       }
     }
     if (tc % h->ngroup == 0) {
