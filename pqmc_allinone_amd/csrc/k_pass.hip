@@ -1009,8 +1009,12 @@ static __device__ __forceinline__ void ps_announce(const ps_args &A, int b0, int
 // so that G[:, q] += G[:, S_A] y[q] is the block's rank-4 updates in turn.
 // Wave w takes columns 4w..4w+3; lane (ql, c, d) = (l / 16, l / 4 % 4, l % 4)
 // forms component c of column 4w+ql, its sums over d reduced in the quad.
-// No workgroup barrier: the waves' columns are independent.  v is kept at
-// sv[q * PS_BS + 4m + c], y at sy[q * PS_BS + 4m + c].
+// No workgroup barrier: the waves' columns are independent.  The vectors
+// stay in registers between the steps: a quad sum leaves component c of v_m
+// in lane (ql, c, d), one shuffle gives each lane the component d that its
+// next products need, and the backward step's row reduction leaves y_m[d]
+// where the next backward products take it.  v goes to sv[q * PS_BS + 4m + c]
+// for the backward half, y to sy[q * PS_BS + 4m + d] for the pivot update.
 static __device__ __forceinline__ void ps_helper_y(const ps_args &A, const ps_helper_hdr &Hh, int k, int a, int ncol,
                                                    int b0, const zd *sC, const int *sS, zd *sv, zd *sy) {
   const int Ns = A.md.Ns, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1020,45 +1024,67 @@ static __device__ __forceinline__ void ps_helper_y(const ps_args &A, const ps_he
   const int *Sk = sS + 4 * A.nb * k;
   const int siteq = qok ? sS[4 * b0 + q] : -1;
   const zd *M = Hh.sAT;
-  for (int m = 0; m < a; ++m) {
-    zd acc = zmk(0.0, 0.0);
-    if (lead) {
-      const int st = Sk[4 * Hh.li[m] + c];
-      acc = zsub(zmk(siteq == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
-    }
-    zd acc1 = zmk(0.0, 0.0);
-    if (qok)
-      for (int mp = 0; mp < m; mp += 2) {
-        acc = zsub(acc, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], sv[q * PS_BS + 4 * mp + d]));
-        if (mp + 1 < m) acc1 = zsub(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], sv[q * PS_BS + 4 * mp + 4 + d]));
+  // after the quad sum lane (ql, c, d) holds component c; lane (ql, d, c) holds d
+  const int tsrc = (lane & 48) | (d << 2) | c;
+  zd vd[PS_NB], yd[PS_NB];  // v_m[d] and y_m[d] of this lane's column
+#pragma unroll
+  for (int m = 0; m < PS_NB; ++m) vd[m] = yd[m] = zmk(0.0, 0.0);
+#pragma unroll
+  for (int m = 0; m < PS_NB; ++m) {
+    if (m < a) {  // uniform
+      zd acc = zmk(0.0, 0.0);
+      if (lead) {
+        const int st = Sk[4 * Hh.li[m] + c];
+        acc = zsub(zmk(siteq == st ? 1.0 : 0.0, 0.0), sC[q * Ns + st]);
       }
-    acc = zadd(acc, acc1);
-    acc.x = quad_sum(acc.x);
-    acc.y = quad_sum(acc.y);
-    if (lead) sv[q * PS_BS + 4 * m + c] = acc;
-    wave_sync();
+      zd acc1 = zmk(0.0, 0.0);
+      if (qok) {
+#pragma unroll
+        for (int mp = 0; mp < m; ++mp) {
+          const zd t = zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], vd[mp]);
+          if (mp & 1)
+            acc1 = zsub(acc1, t);
+          else
+            acc = zsub(acc, t);
+        }
+      }
+      acc = zadd(acc, acc1);
+      acc.x = quad_sum(acc.x);
+      acc.y = quad_sum(acc.y);
+      if (lead) sv[q * PS_BS + 4 * m + c] = acc;  // read by the backward half only
+      vd[m] = shfl_z(acc, tsrc);
+    }
   }
+  wave_sync();
   // z_m = v_m + sum_{m'>m} V[m][m'] y_m', then y_m = X_m z_m (the record holds V above the diagonal)
-  for (int m = a - 1; m >= 0; --m) {
-    zd acc0 = zmk(0.0, 0.0), acc1 = acc0;
-    if (qok) {
-      if (lead) acc0 = sv[q * PS_BS + 4 * m + c];
-      for (int mp = m + 1; mp < a; mp += 2) {
-        acc1 = zadd(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], sy[q * PS_BS + 4 * mp + d]));
-        if (mp + 1 < a) acc0 = zadd(acc0, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], sy[q * PS_BS + 4 * mp + 4 + d]));
+#pragma unroll
+  for (int m = PS_NB - 1; m >= 0; --m) {
+    if (m < a) {  // uniform
+      zd acc0 = zmk(0.0, 0.0), acc1 = acc0;
+      if (qok) {
+        if (lead) acc0 = sv[q * PS_BS + 4 * m + c];
+#pragma unroll
+        for (int mp = m + 1; mp < PS_NB; mp += 2) {
+          if (mp < a) {  // uniform
+            acc1 = zadd(acc1, zmul(M[(4 * m + c) * PS_BS + 4 * mp + d], yd[mp]));
+            if (mp + 1 < a) acc0 = zadd(acc0, zmul(M[(4 * m + c) * PS_BS + 4 * mp + 4 + d], yd[mp + 1 < PS_NB ? mp + 1 : mp]));
+          }
+        }
       }
+      const zd xdc = M[(4 * m + d) * PS_BS + 4 * m + c];  // X_m[d][c]
+      zd zv = zadd(acc0, acc1);
+      zv.x = quad_sum(zv.x);
+      zv.y = quad_sum(zv.y);
+      // lane (ql, c, d) holds z_m[c]; y_m[d] = sum_c X_m[d][c] z_m[c] over the row's lanes of equal d,
+      // which leaves y_m[d] where the next steps use it
+      zd yv = zmul(xdc, zv);
+      yv.x = row4_sum(yv.x);
+      yv.y = row4_sum(yv.y);
+      yd[m] = yv;
+      if (qok && c == 0) sy[q * PS_BS + 4 * m + d] = yv;
     }
-    const zd xdc = M[(4 * m + d) * PS_BS + 4 * m + c];  // X_m[d][c]
-    zd zv = zadd(acc0, acc1);
-    zv.x = quad_sum(zv.x);
-    zv.y = quad_sum(zv.y);
-    // lane (ql, c, d) holds z_m[c]; y_m[d] = sum_c X_m[d][c] z_m[c] over the row's lanes of equal d
-    zd yv = zmul(xdc, zv);
-    yv.x = row4_sum(yv.x);
-    yv.y = row4_sum(yv.y);
-    if (qok && c == 0) sy[q * PS_BS + 4 * m + d] = yv;
-    wave_sync();
   }
+  wave_sync();
 }
 
 // G[r, q] += sum_g G_k[r, S_A(g)] y[q][g] for the own columns, four at a
