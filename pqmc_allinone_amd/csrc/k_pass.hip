@@ -33,7 +33,11 @@
 //    G[S_{k+1}, S_{k+1}] as they were after block k-1 into MFMA
 //    accumulators and apply block k's accepts to them one by one, so that
 //    the window of block k+1 is ready when block k is decided.  The helpers
-//    therefore lag by one block.  Wave 7 prepares the next block's proposals.
+//    therefore lag by one block.  Wave 7 prepares the next block's proposals,
+//    and in the last block those of the next pass's block 0 (ps_args.nxt).
+//  * Pass start: the leader gathers block 0's window from G itself (with the
+//    backward wrap restricted to the block) and takes block 0's proposals
+//    from the previous pass (ps_args.pre) when it formed them.
 //  * Helper h keeps the columns of bph bonds in LDS for the whole pass and
 //    applies block k to them in one step from the record, with the pivot
 //    columns G[:, S_A] from a slot that the owners of block k's columns
