@@ -151,7 +151,8 @@ static __device__ __forceinline__ void qf_scale_col(zd *col, int r0, int m, zd s
 //   W2 = op(T_p) W  (into sPart),
 //   sA -= V_p W2  (16-row tiles over all waves).
 // V_p must be published; T_p is waited for (flag word tflag, none if < 0)
-// after the first product.  Returns false when the launch is aborted.
+// after the first product and its sums.  Returns false when the launch is
+// aborted.
 static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double *sPart, double (*sWr)[17], double (*sWi)[17],
                                 zd *sTp, int p, int ncol, int trans, int tflag, int *sStop) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -211,12 +212,7 @@ static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double
   } else {
     __syncthreads();
   }
-  if (tflag >= 0) {
-    if (!qf_wait(a, tflag, sStop)) return false;  // its barrier also closes the fold
-  } else {
-    __syncthreads();
-  }
-  if (tid < 256) sTp[tid] = Tp[tid];
+  __syncthreads();
   if (tid < 256) {  // W(k, cc): the partial tiles summed, all loads first
     const int k = tid >> 4, cc = tid & 15;
     double xr[8], xi[8];
@@ -229,6 +225,9 @@ static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double
     sWr[k][cc] = ((xr[0] + xr[1]) + (xr[2] + xr[3])) + ((xr[4] + xr[5]) + (xr[6] + xr[7]));
     sWi[k][cc] = ((xi[0] + xi[1]) + (xi[2] + xi[3])) + ((xi[4] + xi[5]) + (xi[6] + xi[7]));
   }
+  // T_p (its barrier also closes the sums)
+  if (tflag >= 0 && !qf_wait(a, tflag, sStop)) return false;
+  if (tid < 256) sTp[tid] = Tp[tid];
   __syncthreads();
   double *w2 = sPart;  // W2, re then im, ld 17 (the partials are consumed)
   if (tid < 256) {
@@ -557,103 +556,97 @@ static __device__ __forceinline__ void qf_panel(const qf_args &a, int b, zd *qf_
     qf_factor_lds(sA, m, j0, nbp, sTau, sBeta, sVn2);
   // 3. publish V_b (0 above row j0 + c, 1 on it, the reflector below it)
   // and raise its flag: the next panels' first products V_b^H A overlap the
-  // forming of T_b
+  // forming of T_b, which starts while the stores drain
   for (int e = tid; e < (m - j0) * nbp; e += QF_THREADS) {
     const int r = j0 + e % (m - j0), c = e / (m - j0);
     const zd v = (r < j0 + c) ? zmk(0.0, 0.0) : (r == j0 + c ? zmk(1.0, 0.0) : sA[r + lm * c]);
     qf_st_z(a.V + r + (size_t)m * (j0 + c), v);
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
-  __syncthreads();
-  if (tid == 0) qf_st(&a.sync[2 + b], a.epoch);
-  // T of zlarft from the Gram matrix Z = V^H V (rows [j0, m))
+  // T of zlarft from the Gram matrix Z = V^H V (rows [j0, m)), its FP64
+  // MFMA products while the stores of V_b drain: Re Z = Vr^T Vr + Vi^T Vi and
+  // X = Vr^T Vi, Im Z = X - X^T
   const int npart = a.npart;
   if (wv < npart) {
     const int li = lane & 15, lk = lane >> 4;
-    double4_t zr = {0.0, 0.0, 0.0, 0.0}, zi = zr;
+    double4_t zr = {0.0, 0.0, 0.0, 0.0}, zx = zr;
     for (int r0 = j0 + 4 * wv; r0 < m; r0 += 4 * npart) {
       const int r = r0 + lk;
       zd v = zmk(0.0, 0.0);
       if (r < m && li < nbp) v = (r < j0 + li) ? zmk(0.0, 0.0) : (r == j0 + li ? zmk(1.0, 0.0) : sA[r + lm * li]);
       zr = mfma_f64(v.x, v.x, zr);
-      zi = mfma_f64(v.x, v.y, zi);
+      zx = mfma_f64(v.x, v.y, zx);
       zr = mfma_f64(v.y, v.y, zr);
-      zi = mfma_f64(-v.y, v.x, zi);
     }
-    double *pr = sPart + (size_t)wv * (QF_PART / 8), *pi = pr + 16 * 17;
+    double *pr = sPart + (size_t)wv * (QF_PART / 8), *px = pr + 16 * 17;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       pr[(lk + 4 * i) * 17 + li] = zr[i];
-      pi[(lk + 4 * i) * 17 + li] = zi[i];
+      px[(lk + 4 * i) * 17 + li] = zx[i];
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores are complete
   __syncthreads();
+  if (tid == 0) qf_st(&a.sync[2 + b], a.epoch);
   if (tid < 256) {  // Z(l, i) into sW2r / sW2i, all loads first
     const int l = tid >> 4, i = tid & 15;
-    double zr[8], zi[8];
+    double zr[8], zx[8], zt[8];
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
       const double *pw = sPart + (size_t)w * (QF_PART / 8);
       zr[w] = w < npart ? pw[l * 17 + i] : 0.0;
-      zi[w] = w < npart ? pw[16 * 17 + l * 17 + i] : 0.0;
+      zx[w] = w < npart ? pw[16 * 17 + l * 17 + i] : 0.0;
+      zt[w] = w < npart ? pw[16 * 17 + i * 17 + l] : 0.0;
     }
     sW2r[l][i] = ((zr[0] + zr[1]) + (zr[2] + zr[3])) + ((zr[4] + zr[5]) + (zr[6] + zr[7]));
-    sW2i[l][i] = ((zi[0] + zi[1]) + (zi[2] + zi[3])) + ((zi[4] + zi[5]) + (zi[6] + zi[7]));
+    sW2i[l][i] = (((zx[0] + zx[1]) + (zx[2] + zx[3])) + ((zx[4] + zx[5]) + (zx[6] + zx[7]))) -
+                 (((zt[0] + zt[1]) + (zt[2] + zt[3])) + ((zt[4] + zt[5]) + (zt[6] + zt[7])));
   }
   __syncthreads();
   // T^-1 = diag(1/tau) + striu(Z), so T = (I + N)^-1 diag(tau) with
   // N = diag(tau) striu(Z) strictly upper (N^16 = 0) and
-  // (I + N)^-1 = (I - N)(I + N^2)(I + N^4)(I + N^8): six 16 x 16 products,
-  // one entry per thread (no 1/tau, so tau = 0 needs no care).
+  // (I + N)^-1 = (I - N)(I + N^2)(I + N^4)(I + N^8): the squarings (threads
+  // 0..255) and the running product W (threads 256..511) side by side, one
+  // entry per thread, in four rounds over ping-pong buffers (no 1/tau, so
+  // tau = 0 needs no care).
   {
-    zd *sPw = reinterpret_cast<zd *>(sPart);  // N, then its powers
-    zd *sWk = sPw + 256;                       // the product so far
-    const int i = tid & 15, k = (tid >> 4) & 15;
-    const bool own = tid < 256;
-    if (own) {
+    zd *sPa = reinterpret_cast<zd *>(sPart), *sPb = sPa + 256;  // powers of N
+    zd *sWa = sPa + 512, *sWb = sPa + 768;                      // W
+    const int i = tid & 15, k = (tid >> 4) & 15, g = tid >> 8;
+    auto mm = [&](const zd *A, const zd *B) {  // (A B)(i, k), ld 16
+      zd pp = zmk(0.0, 0.0);
+#pragma unroll
+      for (int l0 = 0; l0 < 16; l0 += 8) {
+        zd x[8], y[8];
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+          x[l] = A[i + 16 * (l0 + l)];
+          y[l] = B[l0 + l + 16 * k];
+        }
+#pragma unroll
+        for (int l = 0; l < 8; ++l) pp = zadd(pp, zmul(x[l], y[l]));
+      }
+      return pp;
+    };
+    if (g == 0) {
       const zd ti = i < nbp ? sTau[i] : zmk(0.0, 0.0);
       const zd nik = (i < k && k < nbp) ? zmul(ti, zmk(sW2r[i][k], sW2i[i][k])) : zmk(0.0, 0.0);
-      sPw[i + 16 * k] = nik;
-      sWk[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
+      sPa[i + 16 * k] = nik;
+      sWa[i + 16 * k] = zsub(zmk(i == k ? 1.0 : 0.0, 0.0), nik);
     }
     __syncthreads();
-    for (int s = 0; s < 3; ++s) {
-      zd pp = zmk(0.0, 0.0);
-      if (own) {
-#pragma unroll
-        for (int l0 = 0; l0 < 16; l0 += 8) {
-          zd x[8], y[8];
-#pragma unroll
-          for (int l = 0; l < 8; ++l) {
-            x[l] = sPw[i + 16 * (l0 + l)];
-            y[l] = sPw[l0 + l + 16 * k];
-          }
-#pragma unroll
-          for (int l = 0; l < 8; ++l) pp = zadd(pp, zmul(x[l], y[l]));
-        }
-      }
-      __syncthreads();
-      if (own) sPw[i + 16 * k] = pp;
-      __syncthreads();
-      zd wp = zmk(0.0, 0.0);
-      if (own) {
-#pragma unroll
-        for (int l0 = 0; l0 < 16; l0 += 8) {
-          zd x[8], y[8];
-#pragma unroll
-          for (int l = 0; l < 8; ++l) {
-            x[l] = sWk[i + 16 * (l0 + l)];
-            y[l] = sPw[l0 + l + 16 * k];
-          }
-#pragma unroll
-          for (int l = 0; l < 8; ++l) wp = zadd(wp, zmul(x[l], y[l]));
-        }
-      }
-      __syncthreads();
-      if (own) sWk[i + 16 * k] = zadd(sWk[i + 16 * k], wp);
-      __syncthreads();
-    }
+    if (g == 0) sPb[i + 16 * k] = mm(sPa, sPa);  // N^2
+    __syncthreads();
+    if (g == 0) sPa[i + 16 * k] = mm(sPb, sPb);  // N^4
+    if (g == 1) sWb[i + 16 * k] = zadd(sWa[i + 16 * k], mm(sWa, sPb));
+    __syncthreads();
+    if (g == 0) sPb[i + 16 * k] = mm(sPa, sPa);  // N^8
+    if (g == 1) sWa[i + 16 * k] = zadd(sWb[i + 16 * k], mm(sWb, sPa));
+    __syncthreads();
+    if (g == 1) sWb[i + 16 * k] = zadd(sWa[i + 16 * k], mm(sWa, sPb));
+    __syncthreads();
+    const bool own = tid < 256;
+    zd *sWk = sWb;
     // T_b, then its flag once every store of it is complete
     if (own) qf_st_z(a.T + 256 * (size_t)b + i + 16 * k, (k < nbp && i < nbp) ? zmul(sWk[i + 16 * k], sTau[k]) : zmk(0.0, 0.0));
   }
