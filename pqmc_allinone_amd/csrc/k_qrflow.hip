@@ -165,7 +165,9 @@ static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double
   const bool vk = li < npw, ac = li < ncol;
   const int npart = a.npart, nw = min(2 * npart, QF_WAVES);
   if (wv < nw) {
-    double4_t cr0 = {0.0, 0.0, 0.0, 0.0}, ci0 = cr0, cr1 = cr0, ci1 = cr0;
+    // conj(V)^T sA by three real products: P1 = Vr^T Ar, P2 = Vi^T Ai,
+    // P3 = (Vr - Vi)^T (Ar + Ai); re = P1 + P2, im = P3 - P1 + P2
+    double4_t p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
     // chunk q of wave wv at rows r0 + 4 (wv + nw q)
     for (int rb = r0 + 4 * wv; rb < m; rb += 16 * nw) {
       zd vv[4], xx[4];
@@ -176,20 +178,13 @@ static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double
         xx[u] = (r < m && ac) ? sA[r + lm * li] : zmk(0.0, 0.0);
       }
 #pragma unroll
-      for (int u = 0; u < 4; u += 2) {
-        // conj(V)^T sA: re = Vr Ar + Vi Ai, im = Vr Ai - Vi Ar
-        cr0 = mfma_f64(vv[u].x, xx[u].x, cr0);
-        ci0 = mfma_f64(vv[u].x, xx[u].y, ci0);
-        cr1 = mfma_f64(vv[u + 1].x, xx[u + 1].x, cr1);
-        ci1 = mfma_f64(vv[u + 1].x, xx[u + 1].y, ci1);
-        cr0 = mfma_f64(vv[u].y, xx[u].y, cr0);
-        ci0 = mfma_f64(-vv[u].y, xx[u].x, ci0);
-        cr1 = mfma_f64(vv[u + 1].y, xx[u + 1].y, cr1);
-        ci1 = mfma_f64(-vv[u + 1].y, xx[u + 1].x, ci1);
+      for (int u = 0; u < 4; ++u) {
+        p1 = mfma_f64(vv[u].x, xx[u].x, p1);
+        p2 = mfma_f64(vv[u].y, xx[u].y, p2);
+        p3 = mfma_f64(vv[u].x - vv[u].y, xx[u].x + xx[u].y, p3);
       }
     }
-    cr0 += cr1;
-    ci0 += ci1;
+    const double4_t cr0 = p1 + p2, ci0 = (p3 - p1) + p2;
     const int slot = wv < npart ? wv : wv - npart;
     double *pr = sPart + (size_t)slot * (QF_PART / 8), *pi = pr + 16 * 17;
     if (wv >= npart) {
@@ -283,17 +278,18 @@ static __device__ __forceinline__ bool qf_apply(const qf_args &a, zd *sA, double
     for (int q = 0; q < 1; ++q) {
       const int t0 = tb + 16 * QF_WAVES * q;
       if (t0 < m) {  // uniform
-        double4_t pr = {0.0, 0.0, 0.0, 0.0}, pi = pr;
+        // (Vr + i Vi)(Wr + i Wi) by three real products: Q1 = Vr Wr,
+        // Q2 = Vi Wi, Q3 = (Vr + Vi)(Wr + Wi); re = Q1 - Q2, im = Q3 - Q1 - Q2
+        double4_t q1 = {0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
 #pragma unroll
         for (int k4 = 0; k4 < 4; ++k4) {
           if (k4 < nk4) {  // uniform
-            // (Vr + i Vi)(Wr + i Wi)
-            pr = mfma_f64(vv[q][k4].x, w2r[k4], pr);
-            pi = mfma_f64(vv[q][k4].x, w2i[k4], pi);
-            pr = mfma_f64(-vv[q][k4].y, w2i[k4], pr);
-            pi = mfma_f64(vv[q][k4].y, w2r[k4], pi);
+            q1 = mfma_f64(vv[q][k4].x, w2r[k4], q1);
+            q2 = mfma_f64(vv[q][k4].y, w2i[k4], q2);
+            q3 = mfma_f64(vv[q][k4].x + vv[q][k4].y, w2r[k4] + w2i[k4], q3);
           }
         }
+        const double4_t pr = q1 - q2, pi = (q3 - q1) - q2;
         if (ac) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
