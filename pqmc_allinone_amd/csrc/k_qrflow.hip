@@ -464,6 +464,9 @@ static __device__ __forceinline__ void qf_factor_regs(zd *sA, int m, int j0, int
   for (int j = 0; j < nbp; ++j) {
     const int dg = j0 + j;
     if (own && wv > j) {  // uniform per wave
+      // the next pivot's wave carries the step's critical chain: its
+      // instructions issue ahead of the other updates on its SIMD
+      if (wv == j + 1) __builtin_amdgcn_s_setprio(3);
       const zd *vc = sA + (size_t)lm * j;
       zd vs[8];
 #pragma unroll
@@ -494,7 +497,10 @@ static __device__ __forceinline__ void qf_factor_regs(zd *sA, int m, int j0, int
         if (r == dg) col[i] = zsub(col[i], wc);
         else if (r > dg && r < m) col[i] = zsub(col[i], zmul(vs[i], wc));
       }
-      if (wv == j + 1) qf_reflector(col, sA + (size_t)lm * wv, m, dg + 1, j + 1, sTau, sBeta, sVn2);
+      if (wv == j + 1) {
+        qf_reflector(col, sA + (size_t)lm * wv, m, dg + 1, j + 1, sTau, sBeta, sVn2);
+        __builtin_amdgcn_s_setprio(0);
+      }
     }
     __syncthreads();
   }
