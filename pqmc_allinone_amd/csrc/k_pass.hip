@@ -532,6 +532,11 @@ static __device__ __forceinline__ void ps_leader(const ps_args &A, ps_leader_lds
   const int npla = md.npla, nb = A.nb, nblk = A.nblk;
   const size_t tbase = (size_t)md.Nb * (A.t - 1);
   PROF_T0();
+  // the decisions, then the record and flag that release the helpers, are
+  // the block's critical chain: their waves issue ahead of their SIMD
+  // partners (the fold waves)
+  if (wave == 0) __builtin_amdgcn_s_setprio(3);
+  else if (wave == 1) __builtin_amdgcn_s_setprio(2);
   int iW = 0, i11 = 1;
   bool ok = true;
   // a backward pass wraps every column with the exponentials before the
