@@ -17,6 +17,38 @@
 #define GM_T 32
 #define GM_KC 32
 
+// One 32 x 32 tile of C per workgroup (four waves, a 16 x 16 quarter each,
+// v_mfma_f64_16x16x4f64), K in chunks of 32 through LDS.  Every thread holds
+// four elements of A's and four of B's chunk: the loads of the next chunk are
+// in flight while the MFMAs consume the current one.  The complex product
+// takes three real products: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi),
+// re = P1 - P2, im = P3 - P1 - P2.
+template <int OPA, int OPB>
+static __device__ __forceinline__ void zgemm_fetch(int M, int N, int K, const zd *A, int lda, const zd *B, int ldb,
+                                                   int m0, int n0, int k0, zd (&va)[4], zd (&vb)[4]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + 256 * u;
+    int i, k;
+    if (OPA == 0) {
+      i = e % GM_T; k = e / GM_T;
+      va[u] = (m0 + i < M && k0 + k < K) ? A[(m0 + i) + (size_t)lda * (k0 + k)] : zmk(0.0, 0.0);
+    } else {
+      k = e % GM_KC; i = e / GM_KC;
+      va[u] = (m0 + i < M && k0 + k < K) ? zconj(A[(k0 + k) + (size_t)lda * (m0 + i)]) : zmk(0.0, 0.0);
+    }
+    int j;
+    if (OPB == 0) {
+      k = e % GM_KC; j = e / GM_KC;
+      vb[u] = (n0 + j < N && k0 + k < K) ? B[(k0 + k) + (size_t)ldb * (n0 + j)] : zmk(0.0, 0.0);
+    } else {
+      j = e % GM_T; k = e / GM_T;
+      vb[u] = (n0 + j < N && k0 + k < K) ? zconj(B[(n0 + j) + (size_t)ldb * (k0 + k)]) : zmk(0.0, 0.0);
+    }
+  }
+}
+
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zd alpha, const zd *A, int lda,
                                                const zd *B, int ldb, zd beta, zd *C, int ldc) {
@@ -24,47 +56,34 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zd alpha, co
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.x * GM_T, n0 = blockIdx.y * GM_T;
   const int wi = wv >> 1, wj = wv & 1;
-  double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  double4_t p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+  zd va[4], vb[4];
+  zgemm_fetch<OPA, OPB>(M, N, K, A, lda, B, ldb, m0, n0, 0, va, vb);
   for (int k0 = 0; k0 < K; k0 += GM_KC) {
-    for (int e = tid; e < GM_T * GM_KC; e += 256) {
-      int i, k;
-      zd v = zmk(0.0, 0.0);
-      if (OPA == 0) {
-        i = e % GM_T; k = e / GM_T;
-        if (m0 + i < M && k0 + k < K) v = A[(m0 + i) + (size_t)lda * (k0 + k)];
-      } else {
-        k = e % GM_KC; i = e / GM_KC;
-        if (m0 + i < M && k0 + k < K) v = zconj(A[(k0 + k) + (size_t)lda * (m0 + i)]);
-      }
-      sAr[k][i] = v.x;
-      sAi[k][i] = v.y;
-    }
-    for (int e = tid; e < GM_T * GM_KC; e += 256) {
-      int j, k;
-      zd v = zmk(0.0, 0.0);
-      if (OPB == 0) {
-        k = e % GM_KC; j = e / GM_KC;
-        if (n0 + j < N && k0 + k < K) v = B[(k0 + k) + (size_t)ldb * (n0 + j)];
-      } else {
-        j = e % GM_T; k = e / GM_T;
-        if (n0 + j < N && k0 + k < K) v = zconj(B[(n0 + j) + (size_t)ldb * (k0 + k)]);
-      }
-      sBr[k][j] = v.x;
-      sBi[k][j] = v.y;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      const int ia = OPA == 0 ? e % GM_T : e / GM_KC, ka = OPA == 0 ? e / GM_T : e % GM_KC;
+      sAr[ka][ia] = va[u].x;
+      sAi[ka][ia] = va[u].y;
+      const int jb = OPB == 0 ? e / GM_KC : e % GM_T, kb = OPB == 0 ? e % GM_KC : e / GM_T;
+      sBr[kb][jb] = vb[u].x;
+      sBi[kb][jb] = vb[u].y;
     }
     __syncthreads();
+    if (k0 + GM_KC < K) zgemm_fetch<OPA, OPB>(M, N, K, A, lda, B, ldb, m0, n0, k0 + GM_KC, va, vb);
 #pragma unroll
     for (int kk = 0; kk < GM_KC / 4; ++kk) {
       const int krow = kk * 4 + (lane >> 4);
       const double ar = sAr[krow][wi * 16 + (lane & 15)], ai = sAi[krow][wi * 16 + (lane & 15)];
       const double br = sBr[krow][wj * 16 + (lane & 15)], bi = sBi[krow][wj * 16 + (lane & 15)];
-      cr = mfma_f64(ar, br, cr);
-      cr = mfma_f64(-ai, bi, cr);
-      ci = mfma_f64(ar, bi, ci);
-      ci = mfma_f64(ai, br, ci);
+      p1 = mfma_f64(ar, br, p1);
+      p2 = mfma_f64(ai, bi, p2);
+      p3 = mfma_f64(ar + ai, br + bi, p3);
     }
     __syncthreads();
   }
+  const double4_t cr = p1 - p2, ci = (p3 - p1) - p2;
   const bool b0 = (beta.x == 0.0 && beta.y == 0.0);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -575,10 +594,103 @@ __global__ __launch_bounds__(256) void k_trsm_upper(const zd *R, int ldr, int n,
   }
 }
 
+// The same for n <= 16 + 256: thread i < klo keeps row i of R's column
+// panel [klo, klo + 16) in registers, and the next panel's row (with the
+// next diagonal inverse) is loaded while the current one is used, so that
+// no step of the block back substitution waits on a load it issues.
+#define TS_NMAX_R (TS_KB + 256)
+
+struct ts_panel {
+  zd r[TS_KB];
+  zd ri;
+};
+
+static __device__ __forceinline__ void ts_fetch(const zd *R, int ldr, int n, const zd *Rinv, int b, ts_panel &p) {
+  const int tid = threadIdx.x, klo = TS_KB * b, nk = min(TS_KB, n - klo);
+#pragma unroll
+  for (int kk = 0; kk < TS_KB; ++kk) p.r[kk] = (tid < klo && kk < nk) ? R[tid + (size_t)ldr * (klo + kk)] : zmk(0.0, 0.0);
+  p.ri = Rinv[(size_t)b * TS_KB * TS_KB + tid];
+}
+
+// One block step: y_b = Rinv_b z_b, then rows above b less R(:, b) y_b, with
+// panel b in cur; panel b - 1 is fetched into nxt meanwhile.
+static __device__ __forceinline__ void ts_step(const zd *R, int ldr, int n, const zd *Rinv, zd *sZ, zd *sRi, zd *sY, int b,
+                                               const ts_panel &cur, ts_panel &nxt) {
+  const int tid = threadIdx.x, klo = TS_KB * b, nk = min(TS_KB, n - klo);
+  sRi[tid] = cur.ri;
+  __syncthreads();
+  if (b > 0) ts_fetch(R, ldr, n, Rinv, b - 1, nxt);
+  if (tid < TS_COLS * TS_KB) {
+    const int c = tid / TS_KB, i = tid % TS_KB;
+    zd y = zmk(0.0, 0.0);
+    if (i < nk) {
+#pragma unroll
+      for (int k = 0; k < TS_KB; ++k)
+        if (k >= i && k < nk) y = zadd(y, zmul(sRi[i + TS_KB * k], sZ[c * n + klo + k]));
+    }
+    sY[tid] = y;
+  }
+  __syncthreads();
+  if (tid < TS_COLS * TS_KB) {
+    const int c = tid / TS_KB, i = tid % TS_KB;
+    if (i < nk) sZ[c * n + klo + i] = sY[tid];
+  }
+  if (tid < klo) {
+#pragma unroll
+    for (int c = 0; c < TS_COLS; ++c) {
+      zd acc = sZ[c * n + tid];
+#pragma unroll
+      for (int kk = 0; kk < TS_KB; ++kk)
+        if (kk < nk) acc = zsub(acc, zmul(cur.r[kk], sY[c * TS_KB + kk]));
+      sZ[c * n + tid] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_trsm_upper_r(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols,
+                                                      const zd *Rinv) {
+  extern __shared__ zd sm[];
+  zd *sZ = sm;  // TS_COLS x n, column c at sZ[c * n + i]
+  __shared__ zd sRi[TS_KB * TS_KB], sY[TS_COLS * TS_KB];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * TS_COLS;
+  const int ncw = min(TS_COLS, ncols - c0);
+  const int nz = TS_COLS * n;  // <= 5 * 256
+  {
+    zd t[5];
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int e = tid + 256 * u, c = e / n;
+      t[u] = (e < nz && c < ncw) ? Z[e % n + (size_t)ldz * (c0 + c)] : zmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+      if (tid + 256 * u < nz) sZ[tid + 256 * u] = t[u];
+  }
+  const int nblk = (n + TS_KB - 1) / TS_KB;
+  ts_panel pa, pb;
+  ts_fetch(R, ldr, n, Rinv, nblk - 1, pa);
+  for (int b = nblk - 1; b >= 0; b -= 2) {
+    ts_step(R, ldr, n, Rinv, sZ, sRi, sY, b, pa, pb);
+    if (b == 0) break;
+    ts_step(R, ldr, n, Rinv, sZ, sRi, sY, b - 1, pb, pa);
+  }
+  __syncthreads();
+  for (int e = tid; e < ncw * n; e += 256) {
+    const int c = e / n, i = e % n;
+    Z[i + (size_t)ldz * (c0 + c)] = sZ[c * n + i];
+  }
+}
+
 extern "C" hipError_t pqmc_trsm_upper(const zd *R, int ldr, int n, zd *Z, int ldz, int ncols, zd *work,
                                       hipStream_t s) {
   const int nblk = (n + TS_KB - 1) / TS_KB;
   hipLaunchKernelGGL(k_trinv_diag, dim3(nblk), dim3(64), 0, s, R, ldr, n, work);
+  if (n <= TS_NMAX_R) {  // TS_COLS n entries of LDS: at most 17 KB
+    hipLaunchKernelGGL(k_trsm_upper_r, dim3((ncols + TS_COLS - 1) / TS_COLS), dim3(256), sizeof(zd) * TS_COLS * (size_t)n,
+                       s, R, ldr, n, Z, ldz, ncols, (const zd *)work);
+    return hipGetLastError();
+  }
   const size_t lds = sizeof(zd) * (size_t)n * (TS_COLS + TS_KB);
   static std::mutex attr_mu;  // launches may come from several host threads
   static size_t attr = 48 * 1024;
