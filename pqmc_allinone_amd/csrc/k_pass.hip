@@ -433,12 +433,14 @@ static __device__ __forceinline__ void ps_tiles_load(const ps_args &A, int k, ps
   }
 }
 
-// D += a b for complex operands on FP64 MFMA (four real products)
+// D += a b for complex operands on FP64 MFMA by three real products from
+// zero, P1 = ar br, P2 = ai bi, P3 = (ar + ai)(br + bi): re += P1 - P2,
+// im += P3 - P1 - P2 (one MFMA deep instead of two chained)
 static __device__ __forceinline__ void zmfma(double4_t &dr, double4_t &di, zd a, zd b) {
-  dr = mfma_f64(a.x, b.x, dr);
-  di = mfma_f64(a.x, b.y, di);
-  dr = mfma_f64(-a.y, b.y, dr);
-  di = mfma_f64(a.y, b.x, di);
+  const double4_t z = {0.0, 0.0, 0.0, 0.0};
+  const double4_t p1 = mfma_f64(a.x, b.x, z), p2 = mfma_f64(a.y, b.y, z), p3 = mfma_f64(a.x + a.y, b.x + b.y, z);
+  dr += p1 - p2;
+  di += (p3 - p1) - p2;
 }
 
 // Step 1 of accept m: the owners put T10[R, S_j] and T01[S_j, C] in LDS.
@@ -488,7 +490,7 @@ static __device__ __forceinline__ void ps_tiles_store(zd *W, const ps_tiles &T) 
 // diagonal 4 x 4 blocks of the later bonds (the deciding wave updates those),
 // on FP64 MFMA: per 16 x 16 tile of W that meets them, D += U V with U the
 // u_m rows (zero above row o) and V the v_m columns (zero left of column o),
-// complex as four real products of K = 4.  Lane l supplies U[l % 16][l / 16]
+// complex as three real products of K = 4 (zmfma).  Lane l supplies U[l % 16][l / 16]
 // and V[l / 16][l % 16] and holds rows l / 16 + 4 i of column l % 16.  Every
 // entry of W has one writer, in accept order, so the result does not depend
 // on how the two waves interleave.
@@ -511,10 +513,7 @@ static __device__ __forceinline__ void ps_w_offdiag(ps_leader_lds &L, zd *W, int
         dre[i] = t.x;
         dim[i] = t.y;
       }
-      dre = mfma_f64(ua.x, vb.x, dre);
-      dim = mfma_f64(ua.x, vb.y, dim);
-      dre = mfma_f64(-ua.y, vb.y, dre);
-      dim = mfma_f64(ua.y, vb.x, dim);
+      zmfma(dre, dim, ua, vb);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = 16 * rt + lk + 4 * i;
@@ -1172,16 +1171,18 @@ static __device__ __forceinline__ void ps_pivot_update_mfma(const ps_args &A, co
   for (int j = 0; j < NT; ++j) {
     const int t0 = 16 * (wv + NWV * j);
     if (t0 >= Ns) break;  // uniform
-    double4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    // three real products accumulated over the chunks: re = P1 - P2,
+    // im = P3 - P1 - P2
+    double4_t p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       if (4 * ch < n4) {  // uniform
-        cr = mfma_f64(p[j][ch].x, y[ch].x, cr);
-        cr = mfma_f64(-p[j][ch].y, y[ch].y, cr);
-        ci = mfma_f64(p[j][ch].x, y[ch].y, ci);
-        ci = mfma_f64(p[j][ch].y, y[ch].x, ci);
+        p1 = mfma_f64(p[j][ch].x, y[ch].x, p1);
+        p2 = mfma_f64(p[j][ch].y, y[ch].y, p2);
+        p3 = mfma_f64(p[j][ch].x + p[j][ch].y, y[ch].x + y[ch].y, p3);
       }
     }
+    const double4_t cr = p1 - p2, ci = (p3 - p1) - p2;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int row = t0 + lk + 4 * i;
