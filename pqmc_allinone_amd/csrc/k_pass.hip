@@ -400,8 +400,8 @@ static __device__ __forceinline__ void ps_lds_bar_to(const ps_args &A, ps_leader
 // T01 are read again.  Wave w (0..3) keeps the 16 x 16 tile (w / 2, w % 2) of
 // T11, T10 and T01 in MFMA accumulators for the whole block (lane l: rows
 // l/16 + 4i, column l%16); per accept the owners of S_j's column of T10 and
-// row of T01 put them in LDS, and each product is four real FP64 MFMAs of
-// K = 4.  Entries outside the blocks are zero and stay zero.
+// row of T01 put them in LDS, and each product is three real FP64 MFMAs of
+// K = 4 (zmfma).  Entries outside the blocks are zero and stay zero.
 struct ps_tiles {
   double4_t r11, i11, r10, i10, r01, i01;
 };
@@ -1145,7 +1145,7 @@ static __device__ __forceinline__ void ps_pivot_update(const ps_args &A, const p
 
 // The same update on FP64 MFMA when a helper owns 16 columns (four bonds):
 // per 16-row tile, C(16 x 16) += P(16 x n4) Y(n4 x 16) with P the pivot
-// columns G_k[:, S_A] and Y[g][q] = y[q][g], complex as four real products.
+// columns G_k[:, S_A] and Y[g][q] = y[q][g], complex as three real products.
 // Lane l supplies P[t0 + l%16][g0 + l/16] and Y[g0 + l/16][l%16] and holds C
 // rows l/16 + 4i of column l%16.  All pivot loads of a tile are issued
 // before the first product.
