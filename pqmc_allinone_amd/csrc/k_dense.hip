@@ -52,7 +52,9 @@ static __device__ __forceinline__ void zgemm_fetch(int M, int N, int K, const zd
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, zd alpha, const zd *A, int lda,
                                                const zd *B, int ldb, zd beta, zd *C, int ldc) {
-  __shared__ double sAr[GM_KC][GM_T], sAi[GM_KC][GM_T], sBr[GM_KC][GM_T], sBi[GM_KC][GM_T];
+  // rows padded to GM_T + 1: a transposed operand's stores (32 threads along
+  // k) then fall in different banks
+  __shared__ double sAr[GM_KC][GM_T + 1], sAi[GM_KC][GM_T + 1], sBr[GM_KC][GM_T + 1], sBi[GM_KC][GM_T + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.x * GM_T, n0 = blockIdx.y * GM_T;
   const int wi = wv >> 1, wj = wv & 1;
