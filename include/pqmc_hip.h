@@ -240,6 +240,15 @@ int pqmc_update_profile(pqmc_handle *h, int enable, double *out, int32_t *P);
 /* FP64 MFMA operand-layout self check: C(16x16) = A(16x4) B(4x16). */
 int pqmc_mfma_selftest(double *max_abs_err);
 
+/* The stabilization's product and triangular solve on host arrays
+ * (column-major, interleaved complex doubles): which = 0 gives
+ * C = alpha op(A) op(B) + beta C (the ZGEMM calls of cal_g_and_det_T0,
+ * src/mod_evolution.f90:344-350; alpha_beta = {Re a, Im a, Re b, Im b});
+ * which = 1 gives C <- R^-1 C with R = A upper triangular (M x M).  A test
+ * hook: no reference counterpart. */
+int pqmc_dense_check(int which, int opa, int opb, int M, int N, int K, const double *alpha_beta, const double *A,
+                     int lda, const double *B, int ldb, double *C, int ldc);
+
 /* get_expKV_epsoc (src/mod_phonon_field.f90:572-604, selected through the
  * get_expKV pointer for the EPSOC model, :472-496): the spin-up 2x2 bond
  * exponential exp(-dtau H_b), H_b = [[0, h], [conj(h), 0]],

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pqmc_ran2_seed": (None, [C.POINTER(Ran2), C.c_int32]),
     "pqmc_ran2_fill": (None, [C.POINTER(Ran2), C.c_int64, _P]),
     "pqmc_mfma_selftest": (_I, [_P]),
+    "pqmc_dense_check": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _I]),
     "pqmc_expkv_epsoc": (_I, [C.c_double, C.c_int32, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P, C.c_int32, _P]),
     "pqmc_timing": (_I, [_P, _I, _I, _P, _P]),
     "pqmc_update_profile": (_I, [_P, _I, _P, _P]),

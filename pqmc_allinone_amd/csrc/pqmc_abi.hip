@@ -1794,6 +1794,46 @@ extern "C" int pqmc_expkv_epsoc(double dtau, int32_t nfam, const double *kcoe, c
   return done(rc);
 }
 
+// which = 0: C = alpha op(A) op(B) + beta C (pqmc_zgemm; A is lda x (opa ?
+// M : K), B ldb x (opb ? K : N), C ldc x N); which = 1: C <- R^-1 C with R =
+// A upper triangular (M x M, ld lda) and C ldc x N (pqmc_trsm_upper).  Host
+// arrays of interleaved complex doubles, column-major; the library's own
+// device buffers and the null stream.
+extern "C" int pqmc_dense_check(int which, int opa, int opb, int M, int N, int K, const double *alpha_beta,
+                                const double *A, int lda, const double *B, int ldb, double *C, int ldc) {
+  if (which < 0 || which > 1 || M <= 0 || N <= 0 || (which == 0 && K <= 0) || !A || !C || lda < 1 || ldc < M ||
+      (which == 0 && (!B || !alpha_beta || ldb < 1)))
+    return fail(PQMC_EINVAL, "pqmc_dense_check: bad arguments");
+  const size_t na = (size_t)lda * (which == 0 ? (opa ? M : K) : M), nb = which == 0 ? (size_t)ldb * (opb ? K : N) : 0,
+               nc = (size_t)ldc * N, nw = (size_t)256 * ((M + 15) / 16);
+  zd *dA = nullptr, *dB = nullptr, *dC = nullptr, *dW = nullptr;
+  int rc = PQMC_OK;
+  auto run = [&]() -> int {
+    HIPCHK(hipMalloc(&dA, sizeof(zd) * na));
+    HIPCHK(hipMalloc(&dC, sizeof(zd) * nc));
+    HIPCHK(hipMemcpy(dA, A, sizeof(zd) * na, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dC, C, sizeof(zd) * nc, hipMemcpyHostToDevice));
+    if (which == 0) {
+      HIPCHK(hipMalloc(&dB, sizeof(zd) * nb));
+      HIPCHK(hipMemcpy(dB, B, sizeof(zd) * nb, hipMemcpyHostToDevice));
+      const zd alpha = make_double2(alpha_beta[0], alpha_beta[1]), beta = make_double2(alpha_beta[2], alpha_beta[3]);
+      HIPCHK(pqmc_zgemm(opa, opb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, nullptr));
+    } else {
+      HIPCHK(hipMalloc(&dW, sizeof(zd) * nw));
+      HIPCHK(pqmc_trsm_upper(dA, lda, M, dC, ldc, N, dW, nullptr));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(C, dC, sizeof(zd) * nc, hipMemcpyDeviceToHost));
+    return PQMC_OK;
+  };
+  rc = run();
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dC);
+  hipFree(dW);
+  return rc;
+}
+
 extern "C" int pqmc_mfma_selftest(double *max_abs_err) {
   double A[64], B[64], C[256], ref[256];
   for (int i = 0; i < 64; ++i) {
